@@ -1,0 +1,211 @@
+// ctl_device.h -- device-side random numbers and FP32 math of libctl (gfx950).
+//
+// Everything here is built with -ffp-contract=off: a fused multiply-add only
+// happens where fmaf() is written, so the op order is fixed and a host
+// replay of the same algorithm reproduces every value bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ctl {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Per simulation the
+// counter is {block, event, global simulation index, 0}; the key is the
+// 64-bit seed.  An "event" is one leap attempt, one tau-halving retry or one
+// SSA burst; its uniforms are consumed in order, four per Philox block.
+// ---------------------------------------------------------------------------
+struct Rng {
+    uint32_t k0, k1;     // key
+    uint32_t sim;        // global simulation index
+    uint32_t ev;         // current event
+    uint32_t blk;        // next block of the event
+    uint32_t b0, b1, b2, b3;
+    uint32_t pos;        // 4 = buffer empty
+};
+
+__device__ __forceinline__ void philox_block(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                             uint32_t k1, uint32_t& o0, uint32_t& o1, uint32_t& o2,
+                                             uint32_t& o3) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo0 = 0xD2511F53u * c0;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t lo1 = 0xCD9E8D57u * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    o0 = c0; o1 = c1; o2 = c2; o3 = c3;
+}
+
+__device__ __forceinline__ void rng_event(Rng& g, uint32_t ev) {
+    g.ev = ev;
+    g.blk = 0;
+    g.pos = 4;
+}
+
+__device__ __forceinline__ uint32_t rng_u32(Rng& g) {
+    if (g.pos >= 4) {
+        philox_block(g.blk, g.ev, g.sim, 0u, g.k0, g.k1, g.b0, g.b1, g.b2, g.b3);
+        g.blk += 1;
+        g.pos = 0;
+    }
+    const uint32_t p = g.pos;
+    const uint32_t v = p == 0 ? g.b0 : (p == 1 ? g.b1 : (p == 2 ? g.b2 : g.b3));
+    g.pos = p + 1;
+    return v;
+}
+
+// U(0,1] and U[0,1) on the 2^-24 grid.
+__device__ __forceinline__ float rng_oc(Rng& g) { return (float)((rng_u32(g) >> 8) + 1u) * 0x1p-24f; }
+__device__ __forceinline__ float rng_co(Rng& g) { return (float)(rng_u32(g) >> 8) * 0x1p-24f; }
+
+// ---------------------------------------------------------------------------
+// FP32 log and exp: Cody-Waite reduction with ln 2 split into a 16-bit head
+// (exact products with the exponent) and a tail; log(1+f) from the atanh
+// series (coefficients 2/3, 2/5, 2/7, 2/9), exp(r) from its degree-8 Taylor
+// polynomial.  Only +, *, / and explicit fmaf: identical on host and device.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float dlogf(float x) {
+    const uint32_t ix = __float_as_uint(x);
+    int e = (int)(ix >> 23) - 127;
+    uint32_t mb = (ix & 0x007fffffu) | 0x3f800000u;
+    if (mb > 0x3fb504f3u) { mb -= 0x00800000u; e += 1; }
+    const float f = __uint_as_float(mb) - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    float R = fmaf(z, 0x1.c71c72p-3f, 0x1.24924ap-2f);
+    R = fmaf(z, R, 0x1.99999ap-2f);
+    R = fmaf(z, R, 0x1.555556p-1f);
+    R = R * z;
+    const float hfsq = 0.5f * f * f;
+    const float ef = (float)e;
+    return ef * 0x1.62e4p-1f + (f - (hfsq - (s * (hfsq + R) + ef * 0x1.7f7d1cp-20f)));
+}
+
+__device__ __forceinline__ float dexpf(float x) {
+    x = x < -87.0f ? -87.0f : x;
+    x = x > 88.0f ? 88.0f : x;
+    const float kf = rintf(x * 0x1.715476p+0f);
+    const float r = (x - kf * 0x1.62e4p-1f) - kf * 0x1.7f7d1cp-20f;
+    float p = fmaf(r, 0x1.a01a02p-16f, 0x1.a01a02p-13f);
+    p = fmaf(r, p, 0x1.6c16c2p-10f);
+    p = fmaf(r, p, 0x1.111112p-7f);
+    p = fmaf(r, p, 0x1.555556p-5f);
+    p = fmaf(r, p, 0x1.555556p-3f);
+    p = fmaf(r, p, 0.5f);
+    p = fmaf(r, p, 1.0f);
+    p = fmaf(r, p, 1.0f);
+    const int k = (int)kf;
+    return p * __uint_as_float((uint32_t)(k + 127) << 23);
+}
+
+// ---------------------------------------------------------------------------
+// Poisson variates (the reference calls curand_poisson, T.cu:2018, 2040).
+//   lam < 10         inversion by sequential search, one U[0,1); a search
+//                    longer than 63 terms (cumulative stalled under u in
+//                    float) restarts with a fresh uniform
+//   10 <= lam < 2^23 PTRS (Hormann 1993) with the log-pmf from Loader's bd0
+//                    deviance and the Stirling tail, stable in FP32
+//   lam >= 2^23      normal approximation (Marsaglia polar) in FP64
+// recip[k] = 1/k (k < 64) lives in LDS.
+// ---------------------------------------------------------------------------
+__constant__ float c_logfact[10] = {0.0f, 0.0f, 0x1.62e43p-1f, 0x1.cab0cp+0f, 0x1.96ca78p+1f,
+                                     0x1.326644p+2f, 0x1.a51274p+2f, 0x1.10ce2p+3f, 0x1.5358e8p+3f,
+                                     0x1.99a892p+3f};
+
+__device__ __forceinline__ float bd0(float kf, float lam, float diff) {
+    const float sum = kf + lam;
+    if (fabsf(diff) < 0.1f * sum) {
+        const float v = diff / sum;
+        float s = diff * v;
+        float ej = 2.0f * kf * v;
+        const float v2 = v * v;
+        ej = ej * v2; s = s + ej * 0x1.555556p-2f;
+        ej = ej * v2; s = s + ej * 0x1.99999ap-3f;
+        ej = ej * v2; s = s + ej * 0x1.24924ap-3f;
+        ej = ej * v2; s = s + ej * 0x1.c71c72p-4f;
+        ej = ej * v2; s = s + ej * 0x1.745d18p-4f;
+        return s;
+    }
+    return (kf * dlogf(kf / lam) + lam) - kf;
+}
+
+__device__ __noinline__ int32_t poisson_large(Rng& g, float lam) {
+    if (lam >= 8388608.0f) {
+        if (!(lam < 2147483648.0f)) return 2147483647;
+        float u, v, s;
+        do {
+            u = 2.0f * rng_oc(g) - 1.0f;
+            v = 2.0f * rng_oc(g) - 1.0f;
+            s = u * u + v * v;
+        } while (s >= 1.0f || s == 0.0f);
+        const float z = u * sqrtf(-2.0f * dlogf(s) / s);
+        const double kd = floor((double)lam + sqrt((double)lam) * (double)z + 0.5);
+        if (kd < 0.0) return 0;
+        if (kd > 2147483647.0) return 2147483647;
+        return (int32_t)kd;
+    }
+    const float slam = sqrtf(lam);
+    const float loglam = dlogf(lam);
+    const float b = 0.931f + 2.53f * slam;
+    const float a = -0.059f + 0.02483f * b;
+    const float invalpha = 1.1239f + 1.1328f / (b - 3.4f);
+    const float vr = 0.9277f - 3.6224f / (b - 2.0f);
+    const float li = floorf(lam);
+    const float lf = lam - li;
+    for (;;) {
+        const float U = rng_oc(g) - 0.5f;
+        const float V = rng_oc(g);
+        const float us = 0.5f - fabsf(U);
+        if (us < 0.013f && V > us) continue;
+        const float xk = (2.0f * a / us + b) * U + 0.43f;
+        const float d = floorf(xk + lf);
+        if (us >= 0.07f && V <= vr) return (int32_t)li + (int32_t)d;
+        if (d < -li) continue;
+        if (d > 1073741824.0f) continue;
+        const int32_t k = (int32_t)li + (int32_t)d;
+        const float lhs = dlogf(V * invalpha / (a / (us * us) + b));
+        float rhs;
+        if (k < 10) {
+            rhs = (-lam + (float)k * loglam) - c_logfact[k];
+        } else {
+            const float kf = (float)k;
+            const float kinv = 1.0f / kf;
+            const float k2 = kinv * kinv;
+            const float stirl = (0x1.555556p-4f - (0x1.6c16c2p-9f - 0x1.a01a02p-11f * k2) * k2) * kinv;
+            rhs = (-bd0(kf, lam, d - lf) - 0.5f * dlogf(0x1.921fb6p+2f * kf)) - stirl;
+        }
+        if (lhs <= rhs) return k;
+    }
+}
+
+__device__ __forceinline__ int32_t poisson(Rng& g, float lam, const float* __restrict__ recip) {
+    if (!(lam > 0.0f)) return 0;
+    if (lam < 10.0f) {
+        const float p0 = dexpf(-lam);
+        for (;;) {
+            const float u = rng_co(g);
+            float p = p0, F = p0;
+            int k = 0;
+            while (u >= F) {
+                ++k;
+                if (k > 63) break;
+                p = (p * lam) * recip[k];
+                F = F + p;
+            }
+            if (k <= 63) return k;
+        }
+    }
+    return poisson_large(g, lam);
+}
+
+}  // namespace ctl

@@ -1,0 +1,194 @@
+/*
+ * ctl.h -- C ABI of the MI355X-native tau-leaping library (libctl.so).
+ *
+ * cuTauLeaping has no library or FFI interface: its boundary is the process
+ * (kernel.cu:74-436, argv of kernel.cu:78-119) plus the host functions and
+ * kernels that main() calls.  Each entry point below replaces one of those
+ * call sites; the citation names the reference code it stands for.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative status; the codes
+ *    mirror the reference's exit() codes (CTL_E_*).  ctl_last_error()
+ *    returns a thread-local message for the last failure.  No C++
+ *    exception crosses this ABI.
+ *  - The library owns ctl_model / ctl_sim handles; the caller owns every
+ *    output buffer it passes in.
+ *  - A handle is not thread-safe.  Different handles may run concurrently;
+ *    each ctl_sim owns one HIP stream on its device.
+ *  - threads / blocks are semantic (simulation count = threads * blocks,
+ *    input-row selection, fitness grouping), not the launch geometry; the
+ *    library chooses the geometry itself.
+ */
+#ifndef CTL_H
+#define CTL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (kernel.cu / 2phase_n-tau-leaping.cu exit codes). */
+#define CTL_OK            0
+#define CTL_E_ARGS       (-1)   /* bad arguments or malformed input files (T.cu:58-71, 211-214, 244-246) */
+#define CTL_E_DEVICE     (-2)   /* HIP runtime failure or no usable GPU */
+#define CTL_E_NOMEM      (-3)   /* device allocation failed (K.cu:231-234) */
+#define CTL_E_ROWS       (-4)   /* MX_0 has fewer rows than offset + simulations (T.cu:596-599) */
+#define CTL_E_WIDTH      (-5)   /* MX_0 row width differs from the species count (T.cu:628-631) */
+#define CTL_E_NOFILE     (-6)   /* MX_0 missing (T.cu:643-646) */
+#define CTL_E_TTS        (-10)  /* tts_vector does not cover threads*blocks (T.cu:1136-1140) */
+#define CTL_E_STATE      (-11)  /* call out of order (e.g. samples before run) */
+#define CTL_E_LIMIT      (-12)  /* model exceeds a documented size limit */
+
+typedef struct ctl_model ctl_model;
+typedef struct ctl_sim ctl_sim;
+
+/* A model given as arrays (used by generators and tests; the folder loader
+ * builds the same thing). */
+typedef struct {
+    int32_t S, R;
+    const int32_t* left;       /* R*S row-major [r][s], >= 0 (left_side) */
+    const int32_t* right;      /* R*S row-major [r][s], >= 0 (right_side) */
+    const int32_t* feed;       /* S flags, non-zero = feed-locked (M_feed); NULL = none */
+    const uint32_t* x0;        /* x0_rows * S initial counts (MX_0 rows) */
+    int64_t x0_rows;           /* 1 = the row is shared by every simulation */
+    const double* c;           /* c_rows * R rate constants (c_matrix rows) */
+    int64_t c_rows;            /* 1 = the row is shared by every simulation */
+    const double* t_vector;    /* K sample times, t_max = t_vector[K-1] */
+    int32_t K;
+    const int32_t* cs_vector;  /* SS sampled species; NULL = all species */
+    int32_t SS;
+} ctl_model_desc;
+
+typedef struct {
+    int32_t  threads;          /* argv[2]; simulations = threads * blocks */
+    int32_t  blocks;           /* argv[3] */
+    int32_t  gpu;              /* argv[4]: HIP device ordinal */
+    int64_t  offset;           /* argv[5]: first input row; also the Philox stream base */
+    int32_t  force_ssa;        /* argv[9] != "0" */
+    int32_t  nc;               /* critical threshold n_c (T.cuh:75, default 10) */
+    float    eps;              /* error control epsilon (T.cuh:76, default 0.03) */
+    int32_t  ssa_steps;        /* SSA burst length (K.cu:83, default 100) */
+    float    ssa_factor;       /* switch to SSA when tau' < ssa_factor / a0 (T.cu:1935, default 10) */
+    uint64_t seed;             /* Philox key (the reference used time(NULL), K.cu:176) */
+    int32_t  verbose;
+    /* launch geometry (0 = chosen by the library) */
+    int32_t  block_size;       /* threads per workgroup */
+    int32_t  slots;            /* resident simulation slots */
+    int32_t  iters_per_launch; /* leap/SSA iterations per slot per launch */
+    int32_t  ssa_chunk;        /* SSA steps per iteration of a slot */
+} ctl_opts;
+
+typedef struct {
+    uint64_t firings;          /* reactions fired with event time <= t_max */
+    uint64_t leaps;            /* accepted tau-leaps */
+    uint64_t rejections;       /* tau-halving retries */
+    uint64_t switches;         /* leap attempts that fell back to SSA */
+    uint64_t ssa_steps;        /* SSA steps executed (incl. the burst overshoot) */
+    uint64_t poisson;          /* Poisson variates drawn */
+    uint64_t simulations;
+    uint64_t absorbed;         /* simulations that reached a0 = 0 */
+    double   kernel_ms;        /* sum of simulation-kernel durations (HIP events) */
+    double   wall_ms;          /* wall time of ctl_sim_run */
+    int32_t  launches;         /* simulation-kernel launches */
+    int32_t  slots;            /* resident slots used */
+    int32_t  block_size;
+    int32_t  reserved;
+} ctl_stats;
+
+typedef struct {
+    uint64_t firings;          /* per simulation, as in ctl_stats */
+    uint64_t ssa_steps;
+    uint64_t poisson;
+    double   t_end;            /* final clock (t_max + 0.1 after absorption, T.cu:1693) */
+    uint32_t leaps, rejections, switches;
+    uint32_t events;           /* Philox events consumed */
+    uint32_t absorbed;         /* 1 when a0 reached 0 */
+    uint32_t reserved;
+} ctl_sim_stat;
+
+typedef struct {
+    int32_t S, R, K, SS;
+    int32_t nL, nV, nF;        /* non-zeros of the left, var and fear matrices */
+    int64_t x0_rows, c_rows;
+    int64_t row_offset;        /* input row of x0 row 0 (folder loads) */
+    double  t_max;
+} ctl_model_info;
+
+/* ---- options ---------------------------------------------------------- */
+/* Reference defaults: nc = 10, eps = 0.03 (T.cuh:75-76), 100 SSA steps
+ * (K.cu:83), SSA when tau' < 10 / a0 (T.cu:1935). */
+void ctl_opts_default(ctl_opts* o);
+
+/* ---- model ------------------------------------------------------------- */
+/* Parses an input folder: first_pass + read_stoichiometry (T.cu:90-457),
+ * read_quantities_offset (T.cu:570-649), read_parameters_offset
+ * (T.cu:717-782), read_instants (T.cu:932-1053).  Rows [offset, offset+n)
+ * of MX_0 and c_matrix are loaded (n <= 0: every row from offset on). */
+int ctl_model_load_folder(const char* dir, int64_t offset, int64_t n, int verbose, ctl_model** out);
+
+/* Same model from arrays (copies everything). */
+int ctl_model_from_arrays(const ctl_model_desc* d, ctl_model** out);
+
+int ctl_model_info_get(const ctl_model* m, ctl_model_info* out);
+
+/* Compiled structure (read_stoichiometry T.cu:219-335, 402-432): HOR and
+ * HORtype per species, and the left / var (species-major) / fear flats as
+ * (reaction, species, coefficient) triples.  Any pointer may be NULL. */
+int ctl_model_structure(const ctl_model* m, uint8_t* hor, uint8_t* hortype,
+                        int32_t* left_flat, int32_t* rav_flat, int32_t* fear_flat);
+
+/* Fitness inputs ts_rep, p_rep, ts_numtgt, tts_vector, ts_matrix
+ * (read_time_series, T.cu:1056-1261). */
+int ctl_model_load_fitness(ctl_model* m, const char* dir, int32_t threads, int32_t blocks, int verbose);
+
+void ctl_model_destroy(ctl_model* m);
+
+/* ---- simulation -------------------------------------------------------- */
+/* Device allocation and upload: allocate_space (T.cu:2899-2972),
+ * set_symbols (T.cu:865-930), setup_random + PrepareVariables
+ * (T.cu:2974-2988, 1269-1297). */
+int ctl_sim_create(const ctl_model* m, const ctl_opts* o, ctl_sim** out);
+
+/* Runs every simulation to t_max: the host loop of kernel.cu:242-345 with
+ * PreCalculations / EventualSSASteps / LightweightSSA / ParallelVoting
+ * (T.cu:1568-2454, 2990-3002).  Blocking.  May be called again: each call
+ * restarts from the initial state with the same streams. */
+int ctl_sim_run(ctl_sim* s);
+
+/* Sample storage (dev_perthread_storage, T.cu:2966).  layout 0: the
+ * reference's [sample][sampled species][simulation]; layout 1:
+ * [simulation][sample][sampled species].  count must be n*K*SS. */
+int ctl_sim_samples(const ctl_sim* s, uint32_t* out, size_t count, int layout);
+
+/* Device pointer of the [simulation][sample][sampled species] storage (for
+ * callers that keep the samples on the GPU). */
+int ctl_sim_samples_device(const ctl_sim* s, void** dev_ptr, size_t* count);
+
+int ctl_sim_stats(const ctl_sim* s, ctl_stats* out);
+int ctl_sim_sim_stats(const ctl_sim* s, ctl_sim_stat* out, size_t n);
+
+/* Final states and clocks (dev_perthread_X / dev_perthread_T). */
+int ctl_sim_final_state(const ctl_sim* s, int32_t* x /* n*S or NULL */, double* t /* n or NULL */);
+
+/* WriteDynamics2 / PrintDynamics2 (T.cu:3431-3507): one text file
+ * <path_prefix>_<gid> per simulation, or everything on stdout. */
+int ctl_write_dynamics(const ctl_sim* s, const char* path_prefix, int to_stdout);
+
+/* calculateFitness (T.cu:3324-3408): one double per (group, thread),
+ * n / p_rep values.  Needs ctl_model_load_fitness. */
+int ctl_fitness(const ctl_sim* s, double* out, size_t count);
+
+void ctl_sim_destroy(ctl_sim* s);
+
+/* ---- misc -------------------------------------------------------------- */
+const char* ctl_last_error(void);
+const char* ctl_version(void);
+/* Number of HIP devices visible (0 when none); never fails loudly. */
+int ctl_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CTL_H */
