@@ -1,0 +1,114 @@
+"""Seeded synthetic reaction networks for parity tests and benchmarks.
+
+mass_action(N, M, seed)
+    Random mass-action network: every reaction has order k ~ U{0,1,2} with k
+    reactant species drawn uniformly with replacement (a repeated species is a
+    coefficient of 2), m ~ U{1,2} product species drawn uniformly with
+    replacement, rate constant c = 10^U(-6, 0) and initial counts
+    floor(10^U(1, 5)).  A drawn network is rejected (and the next attempt of
+    the same seed is drawn) when its mean-field ODE leaves the int32-safe
+    range: any species above 1e8 before t_end.  The accepted attempt number is
+    returned with the network so the draw can be reproduced.
+
+enzyme_cascade(stages, seed)
+    E_k + S_k <-> C_k -> E_k + S_{k+1}, k = 1..stages, with the last product
+    S_{stages+1}; binding / unbinding log-uniform up to 1e3, catalysis 1e-3,
+    enzymes 5-20, S_1 in [1e5, 1e6].
+
+schlogl()
+    The Schlogl model of the reference repository's schlogl.xml.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _mean_field_ok(left, right, x0, c, t_end, limit=1e8):
+    """Integrates dx/dt = (right-left)^T a(x) with a_j = c_j prod C(x, v)."""
+    from scipy.integrate import solve_ivp
+
+    V = (right - left).astype(np.float64)
+    R, S = left.shape
+
+    def rates(x):
+        a = c.copy()
+        for j in range(R):
+            for s in np.nonzero(left[j])[0]:
+                v = left[j, s]
+                xs = max(x[s], 0.0)
+                if v == 1:
+                    a[j] *= xs
+                else:
+                    a[j] *= xs * max(xs - 1, 0.0) / 2.0
+        return a
+
+    def f(t, x):
+        return V.T @ rates(x)
+
+    def blowup(t, x):
+        return limit - np.max(x)
+    blowup.terminal = True
+
+    sol = solve_ivp(f, (0.0, t_end), x0.astype(np.float64), method="LSODA", events=blowup,
+                    rtol=1e-4, atol=1e-2)
+    if sol.status == -1:
+        return False
+    return bool(np.all(np.isfinite(sol.y)) and np.max(sol.y) < limit)
+
+
+def mass_action(N=100, M=200, seed=1, t_end=50.0, max_attempts=1000, check=True):
+    """Returns dict(left, right, x0, c, attempt)."""
+    for attempt in range(max_attempts):
+        rng = np.random.default_rng([seed, attempt])
+        left = np.zeros((M, N), np.int32)
+        right = np.zeros((M, N), np.int32)
+        for j in range(M):
+            k = rng.integers(0, 3)
+            for s in rng.integers(0, N, size=k):
+                left[j, s] += 1
+            m = rng.integers(1, 3)
+            for s in rng.integers(0, N, size=m):
+                right[j, s] += 1
+        c = 10.0 ** rng.uniform(-6.0, 0.0, size=M)
+        x0 = np.floor(10.0 ** rng.uniform(1.0, 5.0, size=N)).astype(np.uint32)
+        if not check or _mean_field_ok(left, right, x0, c, t_end):
+            return dict(left=left, right=right, x0=x0, c=c, attempt=attempt)
+    raise RuntimeError("no int32-safe network found")
+
+
+def enzyme_cascade(stages=20, seed=4):
+    """Species order: E_1..E_n, S_1..S_{n+1}, C_1..C_n (3n+1 species, 3n reactions)."""
+    rng = np.random.default_rng(seed)
+    n = stages
+    S = 3 * n + 1
+    E = lambda k: k
+    Sub = lambda k: n + k
+    Cx = lambda k: 2 * n + 1 + k
+    left = np.zeros((3 * n, S), np.int32)
+    right = np.zeros((3 * n, S), np.int32)
+    c = np.zeros(3 * n)
+    for k in range(n):
+        b, u, cat = 3 * k, 3 * k + 1, 3 * k + 2
+        left[b, E(k)] = 1; left[b, Sub(k)] = 1; right[b, Cx(k)] = 1          # E + S -> C
+        left[u, Cx(k)] = 1; right[u, E(k)] = 1; right[u, Sub(k)] = 1         # C -> E + S
+        left[cat, Cx(k)] = 1; right[cat, E(k)] = 1; right[cat, Sub(k + 1)] = 1  # C -> E + S'
+        c[b] = 10.0 ** rng.uniform(-3.0, 3.0)
+        c[u] = 10.0 ** rng.uniform(-3.0, 3.0)
+        c[cat] = 1e-3
+    x0 = np.zeros(S, np.uint32)
+    for k in range(n):
+        x0[E(k)] = rng.integers(5, 21)
+    x0[Sub(0)] = int(10.0 ** rng.uniform(5.0, 6.0))
+    return dict(left=left, right=right, x0=x0, c=c)
+
+
+def schlogl():
+    """Schlogl (A, X, B with A, B buffered) as in the reference's schlogl.xml."""
+    left = np.array([[1, 2, 0], [0, 3, 0], [0, 0, 1], [0, 1, 0]], np.int32)
+    right = np.array([[0, 3, 0], [1, 2, 0], [0, 1, 0], [0, 0, 1]], np.int32)
+    return dict(left=left, right=right, x0=np.array([100000, 250, 200000], np.uint32),
+                c=np.array([3e-7, 1e-4, 1e-3, 3.5]), feed=np.array([1, 0, 1], np.int32))
+
+
+def sample_times(K, t_end):
+    return np.array([t_end * (i + 1) / K for i in range(K)], dtype=np.float64)

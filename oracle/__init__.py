@@ -76,7 +76,7 @@ def lib():
         if not os.path.exists(_LIB):
             build()
         L = C.CDLL(_LIB)
-        L.orc_run.argtypes = [C.POINTER(orc_net), C.POINTER(orc_job), C.c_void_p, C.c_void_p]
+        L.orc_run.argtypes = [C.POINTER(orc_net), C.POINTER(orc_job), C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_run.restype = C.c_int
         L.orc_compile.argtypes = [C.POINTER(orc_net), C.POINTER(orc_model)]
         L.orc_compile.restype = C.c_int
@@ -130,8 +130,9 @@ def compile_model(left, right, locked=None) -> dict:
 
 def run(left, right, x0, c, ts, *, locked=None, cs=None, n=None, seed=1, sim_base=0,
         nc=10, eps=0.03, ssa_steps=100, ssa_factor=10.0, force_ssa=False,
-        precision=REPLAY, threads=0):
-    """Runs n simulations; returns (samples[n, K, SS] uint32, stats structured array)."""
+        precision=REPLAY, threads=0, final=False):
+    """Runs n simulations; returns (samples[n, K, SS] uint32, stats structured
+    array) and, with final=True, also the final states [n, S] int32."""
     x0 = np.ascontiguousarray(np.atleast_2d(x0), dtype=np.uint32)
     c = np.ascontiguousarray(np.atleast_2d(c), dtype=np.float64)
     ts = np.ascontiguousarray(ts, dtype=np.float64)
@@ -149,10 +150,11 @@ def run(left, right, x0, c, ts, *, locked=None, cs=None, n=None, seed=1, sim_bas
     out = np.zeros((n, len(ts), len(cs)), dtype=np.uint32)
     st = np.zeros(n, dtype=STAT_DTYPE)
     assert st.dtype.itemsize == C.sizeof(orc_stat)
-    rc = lib().orc_run(C.byref(net), C.byref(job), out.ctypes.data, st.ctypes.data)
+    xf = np.zeros((n, S), dtype=np.int32)
+    rc = lib().orc_run(C.byref(net), C.byref(job), out.ctypes.data, st.ctypes.data, xf.ctypes.data)
     if rc != 0:
         raise ValueError(f"oracle: orc_run failed ({rc})")
-    return out, st
+    return (out, st, xf) if final else (out, st)
 
 
 def philox(ctr, key):

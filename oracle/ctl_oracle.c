@@ -718,7 +718,7 @@ static void run_one(simctx* s) {
     s->st.events = s->evc;
 }
 
-int orc_run(const orc_net* net, const orc_job* J, uint32_t* storage, orc_stat* stats) {
+int orc_run(const orc_net* net, const orc_job* J, uint32_t* storage, orc_stat* stats, int32_t* xfinal) {
     orc_model m;
     if (J->K <= 0 || J->SS < 0 || J->n < 0) return -1;
     for (int sk = 0; sk < J->SS; ++sk)
@@ -756,6 +756,7 @@ int orc_run(const orc_net* net, const orc_job* J, uint32_t* storage, orc_stat* s
             s.g.pos = 4;
             run_one(&s);
             if (stats) stats[i] = s.st;
+            if (xfinal) memcpy(xfinal + (size_t)i * S, s.X, sizeof(int32_t) * S);
             /* run_one may leave X/X1 swapped; keep the buffers this thread owns */
             if (s.X != X) { int32_t* tmp = X; X = X1; X1 = tmp; }
         }

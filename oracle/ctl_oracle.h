@@ -80,8 +80,10 @@ typedef struct {
 int  orc_compile(const orc_net* net, orc_model* m);
 void orc_model_free(orc_model* m);
 
-/* Runs every simulation of the job.  storage is [sim][i][sk] (uint32). */
-int orc_run(const orc_net* net, const orc_job* job, uint32_t* storage, orc_stat* stats);
+/* Runs every simulation of the job.  storage is [sim][i][sk] (uint32);
+ * stats (n entries) and xfinal ([sim][s] final state) may be NULL. */
+int orc_run(const orc_net* net, const orc_job* job, uint32_t* storage, orc_stat* stats,
+            int32_t* xfinal);
 
 /* Sampler entry points for the tests. */
 void  orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
