@@ -37,13 +37,14 @@ HBM_PEAK = 8.0e12                     # bytes/s
 def lane_ops(stats, S, R, nL, nV, nF, SV):
     """Algorithmic VALU lane-ops of a run, from the per-event counters.
     leap attempt (accepted or switched): propensities + criticality (2R + 5 nL),
-    mu/sigma (5 nV), tau candidates (8 S_V), a0c (R); every Poisson draw 40;
-    every update pass 2 nF; SSA step: propensities (2R + 3 nL), search (R),
-    uniforms + log (40), dense update (S)."""
+    mu/sigma (5 nV), tau candidates (8 S_V), a0c (R); every Poisson draw and
+    every exponential gap of the small-rate Poisson process 40; every update
+    pass 2 nF; SSA step: propensities (2R + 3 nL), search (R), uniforms + log
+    (40), dense update (S)."""
     attempts = stats["leaps"] + stats["switches"]
     w = attempts * (2 * R + 5 * nL + 5 * nV + 8 * SV)
     w += stats["leaps"] * R
-    w += stats["poisson"] * 40
+    w += (stats["poisson"] + stats["gaps"]) * 40
     w += (stats["leaps"] + stats["rejections"]) * 2 * nF
     w += stats["ssa_steps"] * (2 * R + 3 * nL + R + 40 + S)
     return float(w)
@@ -105,7 +106,7 @@ def main():
         launches += s["launches"]
         work += lane_ops(s, info.S, info.R, info.nL, info.nV, info.nF, SV)
         print(f"# step {k}: firings={s['firings']} leaps={s['leaps']} ssa_steps={s['ssa_steps']} "
-              f"poisson={s['poisson']} kernel_ms={s['kernel_ms']:.1f} launches={s['launches']}",
+              f"poisson={s['poisson']} gaps={s['gaps']} kernel_ms={s['kernel_ms']:.1f} launches={s['launches']}",
               file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t_start
     last = s
@@ -139,7 +140,8 @@ def main():
                              "peak_GBps": HBM_PEAK / 1e9,
                              "frac": hbm_bytes / (ker_ms / 1000.0 / args.steps) / HBM_PEAK}},
         "counters": {k: last[k] for k in ("firings", "leaps", "rejections", "switches", "ssa_steps",
-                                          "poisson", "absorbed")},
+                                          "poisson", "gaps", "absorbed"), "lds_state": last["lds_state"],
+                     "slots": last["slots"], "block_size": last["block_size"]},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(net, ts, args.cpu_sims)

@@ -79,9 +79,9 @@ class ctl_opts(C.Structure):
 class ctl_stats(C.Structure):
     _fields_ = [("firings", C.c_uint64), ("leaps", C.c_uint64), ("rejections", C.c_uint64),
                 ("switches", C.c_uint64), ("ssa_steps", C.c_uint64), ("poisson", C.c_uint64),
-                ("simulations", C.c_uint64), ("absorbed", C.c_uint64), ("kernel_ms", C.c_double),
-                ("wall_ms", C.c_double), ("launches", C.c_int32), ("slots", C.c_int32),
-                ("block_size", C.c_int32), ("reserved", C.c_int32)]
+                ("gaps", C.c_uint64), ("simulations", C.c_uint64), ("absorbed", C.c_uint64),
+                ("kernel_ms", C.c_double), ("wall_ms", C.c_double), ("launches", C.c_int32),
+                ("slots", C.c_int32), ("block_size", C.c_int32), ("lds_state", C.c_int32)]
 
 
 class ctl_model_info(C.Structure):
@@ -92,7 +92,7 @@ class ctl_model_info(C.Structure):
 
 
 SIM_STAT_DTYPE = np.dtype([("firings", "u8"), ("ssa_steps", "u8"), ("poisson", "u8"),
-                           ("t_end", "f8"), ("leaps", "u4"), ("rejections", "u4"),
+                           ("gaps", "u8"), ("t_end", "f8"), ("leaps", "u4"), ("rejections", "u4"),
                            ("switches", "u4"), ("events", "u4"), ("absorbed", "u4"),
                            ("reserved", "u4")])
 
@@ -260,7 +260,7 @@ class Sim:
 
     def sim_stats(self) -> np.ndarray:
         out = np.zeros(self.n, dtype=SIM_STAT_DTYPE)
-        assert out.dtype.itemsize == 56
+        assert out.dtype.itemsize == 64
         _check(lib().ctl_sim_sim_stats(self._h, out.ctypes.data, self.n))
         return out
 

@@ -52,15 +52,19 @@ __device__ __forceinline__ void rng_event(Rng& g, uint32_t ev) {
     g.pos = 4;
 }
 
+// The block is consumed front to back by rotating the buffer registers (b0 is
+// always the next word), which keeps the whole state in VGPRs.
 __device__ __forceinline__ uint32_t rng_u32(Rng& g) {
     if (g.pos >= 4) {
         philox_block(g.blk, g.ev, g.sim, 0u, g.k0, g.k1, g.b0, g.b1, g.b2, g.b3);
         g.blk += 1;
         g.pos = 0;
     }
-    const uint32_t p = g.pos;
-    const uint32_t v = p == 0 ? g.b0 : (p == 1 ? g.b1 : (p == 2 ? g.b2 : g.b3));
-    g.pos = p + 1;
+    const uint32_t v = g.b0;
+    g.b0 = g.b1;
+    g.b1 = g.b2;
+    g.b2 = g.b3;
+    g.pos += 1;
     return v;
 }
 
@@ -108,6 +112,31 @@ __device__ __forceinline__ float dexpf(float x) {
     return p * __uint_as_float((uint32_t)(k + 127) << 23);
 }
 
+// log(1 - v), v in [0, 1): the series above with f = -v exactly while
+// 1 - v > sqrt(1/2), so small arguments keep their full relative precision.
+__device__ __forceinline__ float dlog1m(float v) {
+    if (v < 0x1.2bec34p-2f) {
+        const float f = -v;
+        const float s = f / (2.0f + f);
+        const float z = s * s;
+        float R = fmaf(z, 0x1.c71c72p-3f, 0x1.24924ap-2f);
+        R = fmaf(z, R, 0x1.99999ap-2f);
+        R = fmaf(z, R, 0x1.555556p-1f);
+        R = R * z;
+        const float hfsq = 0.5f * f * f;
+        return f - (hfsq - s * (hfsq + R));
+    }
+    return dlogf(1.0f - v);
+}
+
+// Exponential(1) variate -log(1 - V): V = r 2^-32 below 2^-8 (exact) and
+// (r >> 8) 2^-24 above, so short gaps keep 32-bit resolution and V < 1.
+__device__ __forceinline__ float exp_rv(Rng& g) {
+    const uint32_t r = rng_u32(g);
+    const float v = r < 0x01000000u ? (float)r * 0x1p-32f : (float)(r >> 8) * 0x1p-24f;
+    return -dlog1m(v);
+}
+
 // ---------------------------------------------------------------------------
 // Poisson variates (the reference calls curand_poisson, T.cu:2018, 2040).
 //   lam < 10         inversion by sequential search, one U[0,1); a search
@@ -139,7 +168,7 @@ __device__ __forceinline__ float bd0(float kf, float lam, float diff) {
     return (kf * dlogf(kf / lam) + lam) - kf;
 }
 
-__device__ __noinline__ int32_t poisson_large(Rng& g, float lam) {
+__device__ __forceinline__ int32_t poisson_large(Rng& g, float lam) {
     if (lam >= 8388608.0f) {
         if (!(lam < 2147483648.0f)) return 2147483647;
         float u, v, s;

@@ -38,7 +38,7 @@ enum : int { MODE_LEAP = 0, MODE_SSA = 1 };
 enum : int { LEAP_DONE = 0, LEAP_OK = 1, LEAP_SWITCH = 2, LEAP_ABSORB = 3 };
 
 struct SlotState {          // SoA, one entry per slot
-    int32_t* sim;           // local simulation index, -1 none
+    int32_t* sim;           // local simulation index, -1 none, -2 queue exhausted
     double* t;
     double* tacc;           // clock advance inside the current SSA burst
     uint32_t* cursor;
@@ -48,10 +48,11 @@ struct SlotState {          // SoA, one entry per slot
     uint32_t* blk;
     uint32_t* pos;
     int32_t* ssa_left;      // steps left in the current SSA burst (0 = none)
-    uint32_t* xb;           // parity of the current state buffer
+    uint32_t* xb;           // parity of the current state buffer (global storage)
     uint64_t* firings;
     uint64_t* ssa_steps;
     uint64_t* poisson;
+    uint64_t* gaps;
     uint32_t* leaps;
     uint32_t* rejections;
     uint32_t* switches;
@@ -60,20 +61,22 @@ struct SlotState {          // SoA, one entry per slot
 struct KParams {
     // model
     int S, R, K, SS;
-    const uint32_t* __restrict__ Lptr;
-    const uint32_t* __restrict__ Lent;
-    const uint32_t* __restrict__ Fptr;
-    const uint32_t* __restrict__ Fent;
-    const uint32_t* __restrict__ Vptr;
-    const uint32_t* __restrict__ Vent;
-    const uint32_t* __restrict__ horw;     // hor | hort << 8
-    const int32_t* __restrict__ cs;
-    const double* __restrict__ ts;
+    const uint32_t* Lptr;
+    const uint32_t* Lent;
+    const uint32_t* Fptr;
+    const uint32_t* Fent;
+    const uint32_t* Vptr;
+    const uint32_t* Vent;
+    const uint32_t* Dptr;     // SSA dependency graph: reactions whose propensity
+    const uint32_t* Dent;     //   reads a species changed by reaction j
+    const uint32_t* horw;     // hor | hort << 8
+    const int32_t* cs;
+    const double* ts;
     double tmax;
-    const float* __restrict__ c_bcast;     // R (broadcast rates) or null
-    const float* __restrict__ c_sims;      // [sim][R] or null
-    const uint32_t* __restrict__ x0_bcast; // S or null
-    const uint32_t* __restrict__ x0_sims;  // [sim][S] or null
+    const float* c_bcast;     // R (rates shared by every simulation) or null
+    const float* c_sims;      // [sim][R] or null
+    const uint32_t* x0_bcast; // S or null
+    const uint32_t* x0_sims;  // [sim][S] or null
     // run
     int n;
     uint32_t sim_base, key0, key1;
@@ -84,21 +87,34 @@ struct KParams {
     int force_ssa;
     int iters;
     int ssa_chunk;
-    // slots
+    // slots (global per-lane storage, [.][ns])
     int ns;
-    int32_t* X;
-    uint32_t* A;
-    float* Cs;
+    int32_t* X;               // [2][S][ns]
+    uint32_t* A;              // [R][ns]
+    float* Cs;                // [R][ns] when rates differ between simulations
     SlotState st;
     // outputs
-    uint32_t* storage;
+    uint32_t* storage;        // [sim][K][SS]
     ctl_sim_stat* simstat;
-    int32_t* xfinal;
+    int32_t* xfinal;          // [sim][S]
     double* tfinal;
     // queue
     uint32_t* next_sim;
     uint32_t* done;
 };
+
+// Model tables are read through constant-address-space pointers: the loads
+// are then invariant, and with the wave-uniform loop indices of the reaction
+// and species loops they become scalar loads (s_load, constant cache) instead
+// of per-lane vector loads.
+typedef const __attribute__((address_space(4))) uint32_t* ctab_u32;
+typedef const __attribute__((address_space(4))) int32_t* ctab_i32;
+typedef const __attribute__((address_space(4))) float* ctab_f32;
+typedef const __attribute__((address_space(4))) double* ctab_f64;
+#define CT_U32(ptr) ((ctab_u32)(ptr))
+#define CT_I32(ptr) ((ctab_i32)(ptr))
+#define CT_F32(ptr) ((ctab_f32)(ptr))
+#define CT_F64(ptr) ((ctab_f64)(ptr))
 
 __constant__ float c_recip[64] = {
     0.0f,      1.0f,      1.0f / 2,  1.0f / 3,  1.0f / 4,  1.0f / 5,  1.0f / 6,  1.0f / 7,
@@ -121,55 +137,73 @@ struct Lane {
     uint32_t evc;
     int ssa_left;
     uint32_t xb;
-    uint64_t firings, ssa_steps, poisson;
+    uint64_t firings, ssa_steps, poisson, gaps;
     uint32_t leaps, rejections, switches;
     uint32_t absorbed;
     Rng g;
 };
 
-__device__ __forceinline__ int32_t* xcur(const KParams& p, const Lane& L) {
-    return p.X + ((size_t)L.xb * p.S) * p.ns + L.slot;
-}
-__device__ __forceinline__ int32_t* xnext(const KParams& p, const Lane& L) {
-    return p.X + ((size_t)(L.xb ^ 1u) * p.S) * p.ns + L.slot;
-}
-__device__ __forceinline__ float rate(const KParams& p, const Lane& L, int j) {
-    return p.c_bcast ? p.c_bcast[j] : p.Cs[(size_t)j * p.ns + L.slot];
-}
+// Where a lane's working arrays live: in LDS (small models, layout
+// [index][lane of the workgroup]) or in the global slot arrays
+// ([index][slot]).  Either way consecutive lanes of a wavefront touch
+// consecutive words.
+struct View {
+    int32_t* X;              // buffer 0 of this lane; buffer 1 is X + xbuf
+    uint32_t* A;
+    float* Cs;
+    size_t st;               // stride between consecutive indices
+    size_t xbuf;             // distance between the two state buffers
+};
+
+__device__ __forceinline__ int32_t* xcur(const View& V, const Lane& L) { return V.X + (L.xb ? V.xbuf : 0); }
+__device__ __forceinline__ int32_t* xnext(const View& V, const Lane& L) { return V.X + (L.xb ? 0 : V.xbuf); }
 __device__ __forceinline__ float aval(uint32_t u) { return __uint_as_float(u & 0x7fffffffu); }
 
-// CalculatePropensity<DEV_FLATTEN_MATRIX_CONST> (T.cu:1443-1470) in FP32 with
-// the criticality test of T.cu:1725-1735 (0 < floor(X/v) < nc) fused in.
-// Writes A[j] and returns a0; a0c accumulates the critical propensities in
-// the same ascending order as the reference's separate loop (T.cu:1945-1952).
-__device__ __forceinline__ float propensities(const KParams& p, const Lane& L, const int32_t* __restrict__ X,
+__device__ __forceinline__ float rate(const KParams& p, const View& V, int j) {
+    return p.c_bcast ? CT_F32(p.c_bcast)[j] : V.Cs[(size_t)j * V.st];
+}
+
+// Propensity of reaction j, CalculatePropensity<DEV_FLATTEN_MATRIX_CONST>
+// (T.cu:1443-1470) in FP32, and its criticality (T.cu:1725-1735: some
+// reactant with 0 < floor(X/v) < nc).  Returns the FP32 bits | crit << 31.
+__device__ __forceinline__ uint32_t prop_one(const KParams& p, const View& V, const int32_t* __restrict__ X, int j,
+                                             float* aout) {
+    float a = rate(p, V, j);
+    bool crit = false;
+    const uint32_t nc = (uint32_t)p.nc;
+    const uint32_t h1 = CT_U32(p.Lptr)[j + 1];
+    for (uint32_t h = CT_U32(p.Lptr)[j]; h < h1; ++h) {
+        const uint32_t e = CT_U32(p.Lent)[h];
+        const int s = (int)(e & 0xffffu);
+        const int v = (int)(e >> 16);
+        const int32_t x = X[(size_t)s * V.st];
+        if (v == 1) {
+            a = a * (float)x;
+        } else if (v == 2) {
+            a = a * ((x < 2) ? 0.0f : (0.5f * (float)x) * (float)(x - 1));
+        } else if (x < v) {
+            a = a * 0.0f;
+        } else {
+            for (int q = 1; q <= v; ++q) a = a * ((float)(x - q + 1) / (float)q);
+        }
+        const uint32_t fl = (uint32_t)x / (uint32_t)v;
+        crit = crit || (fl > 0u && fl < nc);
+    }
+    *aout = a;
+    return __float_as_uint(a) | (crit ? 0x80000000u : 0u);
+}
+
+// All propensities; a0 and a0c summed in ascending reaction order (the
+// reference's loops T.cu:1467-1468 and T.cu:1945-1952).
+__device__ __forceinline__ float propensities(const KParams& p, const View& V, const int32_t* __restrict__ X,
                                               float* a0c) {
     float a0 = 0.0f, ac = 0.0f;
-    const uint32_t nc = (uint32_t)p.nc;
     for (int j = 0; j < p.R; ++j) {
-        float a = rate(p, L, j);
-        bool crit = false;
-        const uint32_t h1 = p.Lptr[j + 1];
-        for (uint32_t h = p.Lptr[j]; h < h1; ++h) {
-            const uint32_t e = p.Lent[h];
-            const int s = (int)(e & 0xffffu);
-            const int v = (int)(e >> 16);
-            const int32_t x = X[(size_t)s * p.ns];
-            if (v == 1) {
-                a = a * (float)x;
-            } else if (v == 2) {
-                a = a * ((x < 2) ? 0.0f : (0.5f * (float)x) * (float)(x - 1));
-            } else if (x < v) {
-                a = a * 0.0f;
-            } else {
-                for (int q = 1; q <= v; ++q) a = a * ((float)(x - q + 1) / (float)q);
-            }
-            const uint32_t fl = (uint32_t)x / (uint32_t)v;
-            crit = crit || (fl > 0u && fl < nc);
-        }
-        p.A[(size_t)j * p.ns + L.slot] = __float_as_uint(a) | (crit ? 0x80000000u : 0u);
+        float a;
+        const uint32_t u = prop_one(p, V, X, j, &a);
+        V.A[(size_t)j * V.st] = u;
         a0 = a0 + a;
-        if (crit) ac = ac + a;
+        if (u & 0x80000000u) ac = ac + a;
     }
     *a0c = ac;
     return a0;
@@ -190,10 +224,10 @@ __device__ __forceinline__ float gfun(uint32_t hw, int32_t x) {
 
 // ABSORB (T.cu:1665-1697, 2165-2196): the remaining samples repeat the last
 // recorded sample.
-__device__ void absorb(const KParams& p, Lane& L, const int32_t* __restrict__ X) {
+__device__ __forceinline__ void absorb(const KParams& p, Lane& L, const View& V, const int32_t* __restrict__ X) {
     uint32_t* out = p.storage + (size_t)L.sim * p.K * p.SS;
     if (L.cursor == 0) {
-        for (int sk = 0; sk < p.SS; ++sk) out[sk] = (uint32_t)X[(size_t)p.cs[sk] * p.ns];
+        for (int sk = 0; sk < p.SS; ++sk) out[sk] = (uint32_t)X[(size_t)CT_I32(p.cs)[sk] * V.st];
         L.cursor = 1;
     }
     const uint32_t* last = out + (size_t)(L.cursor - 1) * p.SS;
@@ -205,31 +239,31 @@ __device__ void absorb(const KParams& p, Lane& L, const int32_t* __restrict__ X)
 }
 
 // One adaptive leap attempt, PreCalculations (T.cu:1568-2093).
-__device__ int leap(const KParams& p, Lane& L, const float* __restrict__ recip) {
+__device__ __forceinline__ int leap(const KParams& p, Lane& L, const View& V, const float* __restrict__ recip) {
     if (L.t >= p.tmax) return LEAP_DONE;
-    int32_t* X = xcur(p, L);
+    int32_t* X = xcur(V, L);
     float a0c;
-    const float a0 = propensities(p, L, X, &a0c);
-    if (a0 == 0.0f) { absorb(p, L, X); return LEAP_ABSORB; }
+    const float a0 = propensities(p, V, X, &a0c);
+    if (a0 == 0.0f) { absorb(p, L, V, X); return LEAP_ABSORB; }
     // tau' over the species-major var flat (T.cu:1860-1922)
     float tau1 = 3.40282347e+38f;
     for (int sp = 0; sp < p.S; ++sp) {
-        const uint32_t h0 = p.Vptr[sp], h1 = p.Vptr[sp + 1];
+        const uint32_t h0 = CT_U32(p.Vptr)[sp], h1 = CT_U32(p.Vptr)[sp + 1];
         if (h0 == h1) continue;
         float mu = 0.0f, sg = 0.0f;
         for (uint32_t h = h0; h < h1; ++h) {
-            const uint32_t e = p.Vent[h];
+            const uint32_t e = CT_U32(p.Vent)[h];
             const int j = (int)(e & 0xffffu);
             const int v = (int)(int16_t)(e >> 16);
-            const uint32_t u = p.A[(size_t)j * p.ns + L.slot];
+            const uint32_t u = V.A[(size_t)j * V.st];
             if (u & 0x80000000u) continue;
             const float a = __uint_as_float(u);
             mu = mu + (float)v * a;
             sg = sg + (float)(v * v) * a;
         }
         if (mu == 0.0f || sg == 0.0f) continue;
-        const int32_t x = X[(size_t)sp * p.ns];
-        const float gg = gfun(p.horw[sp], x);
+        const int32_t x = X[(size_t)sp * V.st];
+        const float gg = gfun(CT_U32(p.horw)[sp], x);
         const float tmp2 = (p.eps * (float)x) / gg;
         const float macs = tmp2 > 1.0f ? tmp2 : 1.0f;
         const float t3 = macs / fabsf(mu);
@@ -240,48 +274,62 @@ __device__ int leap(const KParams& p, Lane& L, const float* __restrict__ recip) 
     if (tau1 < p.ssa_factor / a0) { L.switches++; return LEAP_SWITCH; }   // T.cu:1935-1941
     rng_event(L.g, L.evc++);
     float tau2 = 3.40282347e+38f;
-    if (a0c > 0.0f) tau2 = -dlogf(rng_oc(L.g)) / a0c;                     // T.cu:1962-1965
+    if (a0c > 0.0f) tau2 = exp_rv(L.g) / a0c;                             // T.cu:1962-1965
     float tau = tau1 < tau2 ? tau1 : tau2;
     int pick = -1;
     if (tau1 > tau2) {                                                     // T.cu:1993-2006
         const float w = rng_oc(L.g) * a0c;
-        float acc = aval(p.A[L.slot]);
+        float acc = aval(V.A[0]);
         int r = 0;
         while (w > acc && r < p.R - 1) {
             ++r;
-            const uint32_t u = p.A[(size_t)r * p.ns + L.slot];
+            const uint32_t u = V.A[(size_t)r * V.st];
             if (u & 0x80000000u) acc = acc + aval(u);
         }
         pick = r;
     }
-    int32_t* X1 = xnext(p, L);
+    int32_t* X1 = xnext(V, L);
     uint64_t fired = 0;
     for (int attempt = 0;; ++attempt) {                                    // T.cu:2011-2050
         if (attempt > 0) { rng_event(L.g, L.evc++); L.rejections++; }
-        for (int s = 0; s < p.S; ++s) X1[(size_t)s * p.ns] = X[(size_t)s * p.ns];
+        for (int s = 0; s < p.S; ++s) X1[(size_t)s * V.st] = X[(size_t)s * V.st];
         bool ok = true;
         fired = 0;
-        // test_update (T.cu:1300-1351): reaction-major fear flat, reject on
-        // the first negative partial sum (or an int32 overflow).
+        // Counts: independent Poisson draws for lambda_j >= 1; one shared
+        // rate-1 Poisson process over the running sum of the lambda_j < 1,
+        // walked in reaction order (double precision positions).  Applied
+        // in test_update's order (T.cu:1300-1351): reaction-major fear flat,
+        // reject on the first negative partial sum or an int32 overflow.
+        double cum = 0.0, pos = -1.0;
         for (int j = 0; j < p.R && ok; ++j) {
-            const uint32_t u = p.A[(size_t)j * p.ns + L.slot];
+            const uint32_t u = V.A[(size_t)j * V.st];
             int32_t k;
-            if (!(u & 0x80000000u)) {
-                k = poisson(L.g, tau * __uint_as_float(u), recip);
-                L.poisson++;
-            } else {
+            if (u & 0x80000000u) {
                 k = (attempt == 0 && j == pick) ? 1 : 0;
+            } else {
+                const float lam = tau * __uint_as_float(u);
+                if (lam < 1.0f) {
+                    k = 0;
+                    if (lam > 0.0f) {
+                        if (pos < 0.0) { pos = (double)exp_rv(L.g); L.gaps++; }
+                        cum = cum + (double)lam;
+                        while (pos < cum) { ++k; pos = pos + (double)exp_rv(L.g); L.gaps++; }
+                    }
+                } else {
+                    k = poisson(L.g, lam, recip);
+                    L.poisson++;
+                }
             }
             if (k == 0) continue;
             fired += (uint64_t)k;
-            const uint32_t h1 = p.Fptr[j + 1];
-            for (uint32_t h = p.Fptr[j]; h < h1; ++h) {
-                const uint32_t e = p.Fent[h];
-                const int s = (int)(e & 0xffffu);
+            const uint32_t h1 = CT_U32(p.Fptr)[j + 1];
+            for (uint32_t h = CT_U32(p.Fptr)[j]; h < h1; ++h) {
+                const uint32_t e = CT_U32(p.Fent)[h];
+                const size_t s = (size_t)(e & 0xffffu) * V.st;
                 const int v = (int)(int16_t)(e >> 16);
-                const int64_t nx = (int64_t)X1[(size_t)s * p.ns] + (int64_t)v * (int64_t)k;
+                const int64_t nx = (int64_t)X1[s] + (int64_t)v * (int64_t)k;
                 if (nx < 0 || nx > 2147483647LL) { ok = false; break; }
-                X1[(size_t)s * p.ns] = (int32_t)nx;
+                X1[s] = (int32_t)nx;
             }
         }
         if (ok) break;
@@ -292,12 +340,12 @@ __device__ int leap(const KParams& p, Lane& L, const float* __restrict__ recip) 
     {
         uint32_t i = L.cursor;
         const double tn = L.t + (double)tau;
-        if (i < (uint32_t)p.K && tn >= p.ts[i]) {
+        if (i < (uint32_t)p.K && tn >= CT_F64(p.ts)[i]) {
             uint32_t* out = p.storage + (size_t)L.sim * p.K * p.SS;
-            while (i < (uint32_t)p.K && tn >= p.ts[i]) {
-                const double ti = p.ts[i];
+            while (i < (uint32_t)p.K && tn >= CT_F64(p.ts)[i]) {
+                const double ti = CT_F64(p.ts)[i];
                 for (int sk = 0; sk < p.SS; ++sk) {
-                    const size_t sp = (size_t)p.cs[sk] * p.ns;
+                    const size_t sp = (size_t)CT_I32(p.cs)[sk] * V.st;
                     const int32_t d = X1[sp] - X[sp];
                     uint32_t val = (uint32_t)X[sp];
                     if (d > 0) {
@@ -319,49 +367,62 @@ __device__ int leap(const KParams& p, Lane& L, const float* __restrict__ recip) 
 }
 
 // Up to `steps` direct-method SSA steps of the current burst
-// (EventualSSASteps T.cu:2099-2273, LightweightSSA T.cu:2280-2454).
-// Returns true when the simulation absorbed.
-__device__ bool ssa_steps(const KParams& p, Lane& L, int steps) {
-    int32_t* X = xcur(p, L);
+// (EventualSSASteps T.cu:2099-2273, LightweightSSA T.cu:2280-2454).  The
+// reference recomputes every propensity each step; here only the reactions
+// that read a species the fired reaction changed are recomputed (dependency
+// graph), which yields the same values, and a0 is re-summed in reaction
+// order each step.  Returns true when the simulation absorbed.
+__device__ __forceinline__ bool ssa_steps(const KParams& p, Lane& L, const View& V, int steps) {
+    int32_t* X = xcur(V, L);
+    float dummy;
+    float a0 = propensities(p, V, X, &dummy);
     for (int st = 0; st < steps; ++st) {
-        float dummy;
-        const float a0 = propensities(p, L, X, &dummy);
-        if (a0 == 0.0f) { absorb(p, L, X); return true; }
+        if (a0 == 0.0f) { absorb(p, L, V, X); return true; }
         const float r1 = rng_oc(L.g) * a0;
-        const float r2 = rng_oc(L.g);
         int j = 0;
-        float acc = aval(p.A[L.slot]);
+        float acc = aval(V.A[0]);
         while (r1 > acc && j < p.R - 1) {
             ++j;
-            acc = acc + aval(p.A[(size_t)j * p.ns + L.slot]);
+            acc = acc + aval(V.A[(size_t)j * V.st]);
         }
-        const float dt = -dlogf(r2) / a0;
+        const float dt = exp_rv(L.g) / a0;
         L.tacc = L.tacc + (double)dt;
         const double tn = L.t + L.tacc;
         uint32_t i = L.cursor;
-        if (i < (uint32_t)p.K && tn > p.ts[i]) {
+        if (i < (uint32_t)p.K && tn > CT_F64(p.ts)[i]) {
             uint32_t* out = p.storage + (size_t)L.sim * p.K * p.SS;
-            while (i < (uint32_t)p.K && tn > p.ts[i]) {
+            while (i < (uint32_t)p.K && tn > CT_F64(p.ts)[i]) {
                 for (int sk = 0; sk < p.SS; ++sk)
-                    out[(size_t)i * p.SS + sk] = (uint32_t)X[(size_t)p.cs[sk] * p.ns];
+                    out[(size_t)i * p.SS + sk] = (uint32_t)X[(size_t)CT_I32(p.cs)[sk] * V.st];
                 ++i;
             }
             L.cursor = i;
         }
-        const uint32_t h1 = p.Fptr[j + 1];
-        for (uint32_t h = p.Fptr[j]; h < h1; ++h) {
-            const uint32_t e = p.Fent[h];
-            const size_t s = (size_t)(e & 0xffffu) * p.ns;
+        const uint32_t h1 = CT_U32(p.Fptr)[j + 1];
+        for (uint32_t h = CT_U32(p.Fptr)[j]; h < h1; ++h) {
+            const uint32_t e = CT_U32(p.Fent)[h];
+            const size_t s = (size_t)(e & 0xffffu) * V.st;
             const int v = (int)(int16_t)(e >> 16);
             X[s] = (int32_t)((uint32_t)X[s] + (uint32_t)v);
         }
         L.ssa_steps++;
         if (tn <= p.tmax) L.firings++;
+        if (st + 1 < steps) {
+            // j is per lane: walk its dependency list (not wave-uniform)
+            const uint32_t d1 = p.Dptr[j + 1];
+            for (uint32_t d = p.Dptr[j]; d < d1; ++d) {
+                const int r = (int)p.Dent[d];
+                float a;
+                V.A[(size_t)r * V.st] = prop_one(p, V, X, r, &a);
+            }
+            a0 = 0.0f;
+            for (int r = 0; r < p.R; ++r) a0 = a0 + aval(V.A[(size_t)r * V.st]);
+        }
     }
     return false;
 }
 
-__device__ void load_lane(const KParams& p, Lane& L) {
+__device__ __forceinline__ void load_lane(const KParams& p, Lane& L) {
     const int s = L.slot;
     L.sim = p.st.sim[s];
     L.g.k0 = p.key0;
@@ -378,6 +439,7 @@ __device__ void load_lane(const KParams& p, Lane& L) {
     L.firings = p.st.firings[s];
     L.ssa_steps = p.st.ssa_steps[s];
     L.poisson = p.st.poisson[s];
+    L.gaps = p.st.gaps[s];
     L.leaps = p.st.leaps[s];
     L.rejections = p.st.rejections[s];
     L.switches = p.st.switches[s];
@@ -387,12 +449,18 @@ __device__ void load_lane(const KParams& p, Lane& L) {
     L.g.pos = p.st.pos[s];
     if (L.g.pos < 4u) {   // regenerate the partially consumed block of an SSA burst
         philox_block(L.g.blk - 1u, L.g.ev, L.g.sim, 0u, L.g.k0, L.g.k1, L.g.b0, L.g.b1, L.g.b2, L.g.b3);
+        for (uint32_t i = 0; i < L.g.pos; ++i) {   // drop the words already consumed
+            L.g.b0 = L.g.b1;
+            L.g.b1 = L.g.b2;
+            L.g.b2 = L.g.b3;
+        }
     }
 }
 
-__device__ void store_lane(const KParams& p, const Lane& L) {
+__device__ __forceinline__ void store_lane(const KParams& p, const Lane& L) {
     const int s = L.slot;
     p.st.sim[s] = L.sim;
+    if (L.sim < 0) return;
     p.st.t[s] = L.t;
     p.st.tacc[s] = L.tacc;
     p.st.cursor[s] = L.cursor;
@@ -403,6 +471,7 @@ __device__ void store_lane(const KParams& p, const Lane& L) {
     p.st.firings[s] = L.firings;
     p.st.ssa_steps[s] = L.ssa_steps;
     p.st.poisson[s] = L.poisson;
+    p.st.gaps[s] = L.gaps;
     p.st.leaps[s] = L.leaps;
     p.st.rejections[s] = L.rejections;
     p.st.switches[s] = L.switches;
@@ -411,9 +480,15 @@ __device__ void store_lane(const KParams& p, const Lane& L) {
     p.st.pos[s] = L.g.pos;
 }
 
+// Per-simulation rates into the lane's rate array (rows that differ).
+__device__ __forceinline__ void load_rates(const KParams& p, const View& V, int sim) {
+    if (!p.c_bcast)
+        for (int j = 0; j < p.R; ++j) V.Cs[(size_t)j * V.st] = p.c_sims[(size_t)sim * p.R + j];
+}
+
 // Takes the next simulation from the queue and initialises the lane
 // (PrepareVariables, T.cu:1269-1297).  Returns false when the queue is empty.
-__device__ bool fetch(const KParams& p, Lane& L) {
+__device__ __forceinline__ bool fetch(const KParams& p, Lane& L, const View& V) {
     const uint32_t q = atomicAdd(p.next_sim, 1u);
     if (q >= (uint32_t)p.n) { L.sim = -2; return false; }
     L.sim = (int)q;
@@ -423,69 +498,96 @@ __device__ bool fetch(const KParams& p, Lane& L) {
     L.evc = 0;
     L.ssa_left = 0;
     L.xb = 0;
-    L.firings = L.ssa_steps = L.poisson = 0;
+    L.firings = L.ssa_steps = L.poisson = L.gaps = 0;
     L.leaps = L.rejections = L.switches = 0;
     L.absorbed = 0;
     L.mode = p.force_ssa ? MODE_SSA : MODE_LEAP;
     L.g.sim = p.sim_base + q;
     L.g.pos = 4;
-    int32_t* X = xcur(p, L);
+    int32_t* X = xcur(V, L);
     for (int s = 0; s < p.S; ++s)
-        X[(size_t)s * p.ns] = (int32_t)(p.x0_bcast ? p.x0_bcast[s] : p.x0_sims[(size_t)q * p.S + s]);
-    if (!p.c_bcast)
-        for (int j = 0; j < p.R; ++j) p.Cs[(size_t)j * p.ns + L.slot] = p.c_sims[(size_t)q * p.R + j];
+        X[(size_t)s * V.st] = (int32_t)(p.x0_bcast ? CT_U32(p.x0_bcast)[s] : p.x0_sims[(size_t)q * p.S + s]);
+    load_rates(p, V, (int)q);
     return true;
 }
 
-__device__ void finish(const KParams& p, Lane& L) {
-    const int32_t* X = xcur(p, L);
+__device__ __forceinline__ void finish(const KParams& p, Lane& L, const View& V) {
+    const int32_t* X = xcur(V, L);
     ctl_sim_stat st;
     st.firings = L.firings;
+    st.ssa_steps = L.ssa_steps;
+    st.poisson = L.poisson;
+    st.gaps = L.gaps;
+    st.t_end = L.t;
     st.leaps = L.leaps;
     st.rejections = L.rejections;
     st.switches = L.switches;
     st.events = L.evc;
-    st.ssa_steps = L.ssa_steps;
-    st.poisson = L.poisson;
-    st.t_end = L.t;
     st.absorbed = L.absorbed;
     st.reserved = 0;
     p.simstat[L.sim] = st;
     p.tfinal[L.sim] = L.t;
-    for (int s = 0; s < p.S; ++s) p.xfinal[(size_t)L.sim * p.S + s] = X[(size_t)s * p.ns];
+    for (int s = 0; s < p.S; ++s) p.xfinal[(size_t)L.sim * p.S + s] = X[(size_t)s * V.st];
     atomicAdd(p.done, 1u);
     L.sim = -1;
 }
 
+// One slot per lane; kLDS keeps the lane's state, trial state, propensities
+// (and per-simulation rates) in LDS for the duration of the launch, copied
+// from / to the global slot arrays at entry / exit.
+template <bool kLDS>
 __global__ __launch_bounds__(256) void tau_leap_kernel(KParams p) {
+    extern __shared__ uint32_t smem[];
     __shared__ float recip[64];
     for (int i = threadIdx.x; i < 64; i += blockDim.x) recip[i] = c_recip[i];
     __syncthreads();
     Lane L;
     L.slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (L.slot >= p.ns) return;
+    View V;
+    if (kLDS) {
+        const size_t B = blockDim.x;
+        int32_t* base = (int32_t*)smem + threadIdx.x;
+        V.X = base;
+        V.xbuf = (size_t)p.S * B;
+        V.A = (uint32_t*)(base + 2 * V.xbuf);
+        V.Cs = (float*)(V.A + (size_t)p.R * B);
+        V.st = B;
+    } else {
+        V.X = p.X + L.slot;
+        V.xbuf = (size_t)p.S * p.ns;
+        V.A = p.A + L.slot;
+        V.Cs = p.Cs ? p.Cs + L.slot : nullptr;
+        V.st = (size_t)p.ns;
+    }
     load_lane(p, L);
     if (L.sim == -2) return;             // queue was exhausted for this slot
+    if (kLDS && L.sim >= 0) {            // resume: state from the global slot arrays
+        const int32_t* G = p.X + (L.xb ? (size_t)p.S * p.ns : 0) + L.slot;
+        for (int s = 0; s < p.S; ++s) V.X[(size_t)s * V.st] = G[(size_t)s * p.ns];
+        L.xb = 0;
+        load_rates(p, V, L.sim);
+    }
     for (int it = 0; it < p.iters; ++it) {
-        if (L.sim < 0 && !fetch(p, L)) break;
+        if (L.sim < 0 && !fetch(p, L, V)) break;
         if (L.mode == MODE_LEAP) {
-            const int r = leap(p, L, recip);
+            const int r = leap(p, L, V, recip);
             if (r == LEAP_DONE || r == LEAP_ABSORB) {
-                finish(p, L);
+                finish(p, L, V);
             } else if (r == LEAP_SWITCH) {
                 L.mode = MODE_SSA;
             }
         } else {
             if (L.ssa_left == 0) {       // open a burst
-                if (p.force_ssa && L.t >= p.tmax) { finish(p, L); continue; }
+                if (p.force_ssa && L.t >= p.tmax) { finish(p, L, V); continue; }
                 rng_event(L.g, L.evc++);
                 L.tacc = 0.0;
                 L.ssa_left = p.ssa_steps;
             }
             const int n = L.ssa_left < p.ssa_chunk ? L.ssa_left : p.ssa_chunk;
-            if (ssa_steps(p, L, n)) {
+            if (ssa_steps(p, L, V, n)) {
                 L.ssa_left = 0;
-                finish(p, L);
+                finish(p, L, V);
                 continue;
             }
             L.ssa_left -= n;
@@ -495,6 +597,12 @@ __global__ __launch_bounds__(256) void tau_leap_kernel(KParams p) {
                 if (!p.force_ssa) L.mode = MODE_LEAP;
             }
         }
+    }
+    if (kLDS && L.sim >= 0) {            // suspend: current state to the global slot arrays
+        const int32_t* X = xcur(V, L);
+        int32_t* G = p.X + L.slot;
+        for (int s = 0; s < p.S; ++s) G[(size_t)s * p.ns] = X[(size_t)s * V.st];
+        L.xb = 0;
     }
     store_lane(p, L);
 }
@@ -554,6 +662,8 @@ struct ctl_sim {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int n = 0, ns = 0, block = 256, grid = 0;
+    bool lds = false;
+    size_t smem = 0;
     KParams kp{};
     std::vector<void*> allocs;
     bool ran = false;
@@ -660,10 +770,21 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     hipDeviceProp_t prop;
     e = hipGetDeviceProperties(&prop, s->device);
     if (e != hipSuccess) return bail(hip_fail(e, "hipGetDeviceProperties"));
-    s->block = o->block_size > 0 ? o->block_size : 256;
+    const int S = m.S, R = m.R, K = (int)m.ts.size(), SS = (int)m.cs.size();
+    // Launch geometry.  Per-lane working set (state, trial state,
+    // propensities, per-simulation rates) in LDS when a 64-lane wavefront's
+    // share stays within 16 KB (>= 8 wavefronts per CU); otherwise in the
+    // global slot arrays with 256-lane workgroups.
+    const size_t lane_words = (size_t)2 * S + R + (m.c_rows == 1 ? 0 : R);
+    const bool lds_auto = lane_words * 4 * 64 <= 16384;
+    s->lds = o->block_size < 0 ? false : lds_auto;
+    s->block = o->block_size > 0 ? o->block_size : (s->lds ? 64 : 256);
     if (s->block % 64 != 0 || s->block > 256) return bail(ctl::fail(CTL_E_ARGS, "block_size must be 64..256, multiple of 64"));
+    s->smem = s->lds ? lane_words * 4 * s->block : 0;
+    if (s->smem > 65536) { s->lds = false; s->smem = 0; }
+    const void* kfn = s->lds ? (const void*)tau_leap_kernel<true> : (const void*)tau_leap_kernel<false>;
     int per_cu = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)tau_leap_kernel, s->block, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, s->block, s->smem);
     if (e != hipSuccess || per_cu <= 0) per_cu = 1;
     int ns = o->slots > 0 ? o->slots : prop.multiProcessorCount * per_cu * s->block;
     if (ns > n) ns = ((n + s->block - 1) / s->block) * s->block;
@@ -671,23 +792,44 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     s->grid = (ns + s->block - 1) / s->block;
 
     KParams& p = s->kp;
-    const int S = m.S, R = m.R, K = (int)m.ts.size(), SS = (int)m.cs.size();
     p.S = S; p.R = R; p.K = K; p.SS = SS;
     int rc = 0;
     std::vector<uint32_t> horw(S);
     for (int i = 0; i < S; ++i) horw[i] = (uint32_t)m.hor[i] | ((uint32_t)m.hort[i] << 8);
     std::vector<float> cf(m.c.size());
     for (size_t i = 0; i < m.c.size(); ++i) cf[i] = (float)m.c[i];
-    uint32_t *Lptr, *Lent, *Fptr, *Fent, *Vptr, *Vent, *hw;
+    // SSA dependency graph: for each reaction j, the reactions with a
+    // reactant among the species j changes (its fear entries), ascending.
+    std::vector<uint32_t> Dptr(R + 1, 0), Dent;
+    {
+        std::vector<std::vector<uint32_t>> readers(S);
+        for (int r = 0; r < R; ++r)
+            for (uint32_t h = m.Lptr[r]; h < m.Lptr[r + 1]; ++h) readers[m.Lent[h] & 0xffffu].push_back((uint32_t)r);
+        std::vector<char> mark(R, 0);
+        std::vector<uint32_t> list;
+        for (int j = 0; j < R; ++j) {
+            Dptr[j] = (uint32_t)Dent.size();
+            list.clear();
+            for (uint32_t h = m.Fptr[j]; h < m.Fptr[j + 1]; ++h)
+                for (uint32_t r : readers[m.Fent[h] & 0xffffu])
+                    if (!mark[r]) { mark[r] = 1; list.push_back(r); }
+            std::sort(list.begin(), list.end());
+            for (uint32_t r : list) { mark[r] = 0; Dent.push_back(r); }
+        }
+        Dptr[R] = (uint32_t)Dent.size();
+    }
+    uint32_t *Lptr, *Lent, *Fptr, *Fent, *Vptr, *Vent, *Dp, *De, *hw;
     int32_t* cs;
     double* ts;
     if ((rc = upload(s, &Lptr, m.Lptr.data(), m.Lptr.size())) || (rc = upload(s, &Lent, m.Lent.data(), m.Lent.size())) ||
         (rc = upload(s, &Fptr, m.Fptr.data(), m.Fptr.size())) || (rc = upload(s, &Fent, m.Fent.data(), m.Fent.size())) ||
         (rc = upload(s, &Vptr, m.Vptr.data(), m.Vptr.size())) || (rc = upload(s, &Vent, m.Vent.data(), m.Vent.size())) ||
+        (rc = upload(s, &Dp, Dptr.data(), Dptr.size())) || (rc = upload(s, &De, Dent.data(), Dent.size())) ||
         (rc = upload(s, &hw, horw.data(), horw.size())) || (rc = upload(s, &cs, m.cs.data(), m.cs.size())) ||
         (rc = upload(s, &ts, m.ts.data(), m.ts.size())))
         return bail(rc);
     p.Lptr = Lptr; p.Lent = Lent; p.Fptr = Fptr; p.Fent = Fent; p.Vptr = Vptr; p.Vent = Vent;
+    p.Dptr = Dp; p.Dent = De;
     p.horw = hw; p.cs = cs; p.ts = ts; p.tmax = m.ts.back();
     float* cdev;
     uint32_t* xdev;
@@ -717,7 +859,8 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
         (rc = dalloc(s, &st.cursor, ns)) || (rc = dalloc(s, &st.mode, ns)) || (rc = dalloc(s, &st.evc, ns)) ||
         (rc = dalloc(s, &st.ev, ns)) || (rc = dalloc(s, &st.blk, ns)) || (rc = dalloc(s, &st.pos, ns)) ||
         (rc = dalloc(s, &st.ssa_left, ns)) || (rc = dalloc(s, &st.xb, ns)) || (rc = dalloc(s, &st.firings, ns)) ||
-        (rc = dalloc(s, &st.ssa_steps, ns)) || (rc = dalloc(s, &st.poisson, ns)) || (rc = dalloc(s, &st.leaps, ns)) ||
+        (rc = dalloc(s, &st.ssa_steps, ns)) || (rc = dalloc(s, &st.poisson, ns)) || (rc = dalloc(s, &st.gaps, ns)) ||
+        (rc = dalloc(s, &st.leaps, ns)) ||
         (rc = dalloc(s, &st.rejections, ns)) || (rc = dalloc(s, &st.switches, ns)))
         return bail(rc);
     if ((rc = dalloc(s, &p.storage, (size_t)n * K * SS)) || (rc = dalloc(s, &p.simstat, n)) ||
@@ -727,6 +870,7 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     p.done = p.next_sim + 1;
     s->stats.slots = ns;
     s->stats.block_size = s->block;
+    s->stats.lds_state = s->lds ? 1 : 0;
     *out = s;
     return 0;
 }
@@ -749,7 +893,10 @@ int ctl_sim_run(ctl_sim* s) {
     float total_ms = 0.0f;
     while (done < (uint32_t)s->n) {
         HIPCHK(hipEventRecord(s->ev0, s->stream), "hipEventRecord");
-        hipLaunchKernelGGL(tau_leap_kernel, dim3(s->grid), dim3(s->block), 0, s->stream, p);
+        if (s->lds)
+            hipLaunchKernelGGL(tau_leap_kernel<true>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p);
+        else
+            hipLaunchKernelGGL(tau_leap_kernel<false>, dim3(s->grid), dim3(s->block), 0, s->stream, p);
         HIPCHK(hipGetLastError(), "tau_leap_kernel launch");
         HIPCHK(hipEventRecord(s->ev1, s->stream), "hipEventRecord");
         HIPCHK(hipMemcpyAsync(&done, p.done, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream), "hipMemcpy D2H");
@@ -766,10 +913,10 @@ int ctl_sim_run(ctl_sim* s) {
     // totals
     std::vector<ctl_sim_stat> st(s->n);
     HIPCHK(hipMemcpy(st.data(), p.simstat, sizeof(ctl_sim_stat) * s->n, hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    S.firings = S.leaps = S.rejections = S.switches = S.ssa_steps = S.poisson = S.absorbed = 0;
+    S.firings = S.leaps = S.rejections = S.switches = S.ssa_steps = S.poisson = S.gaps = S.absorbed = 0;
     for (const auto& x : st) {
         S.firings += x.firings; S.leaps += x.leaps; S.rejections += x.rejections; S.switches += x.switches;
-        S.ssa_steps += x.ssa_steps; S.poisson += x.poisson;
+        S.ssa_steps += x.ssa_steps; S.poisson += x.poisson; S.gaps += x.gaps;
         S.absorbed += x.absorbed;
     }
     S.simulations = (uint64_t)s->n;

@@ -86,7 +86,8 @@ typedef struct {
     uint64_t rejections;       /* tau-halving retries */
     uint64_t switches;         /* leap attempts that fell back to SSA */
     uint64_t ssa_steps;        /* SSA steps executed (incl. the burst overshoot) */
-    uint64_t poisson;          /* Poisson variates drawn */
+    uint64_t poisson;          /* individual Poisson variates drawn (lambda >= 1) */
+    uint64_t gaps;             /* exponential gaps of the small-lambda Poisson process */
     uint64_t simulations;
     uint64_t absorbed;         /* simulations that reached a0 = 0 */
     double   kernel_ms;        /* sum of simulation-kernel durations (HIP events) */
@@ -94,13 +95,14 @@ typedef struct {
     int32_t  launches;         /* simulation-kernel launches */
     int32_t  slots;            /* resident slots used */
     int32_t  block_size;
-    int32_t  reserved;
+    int32_t  lds_state;        /* 1 when per-lane state lives in LDS */
 } ctl_stats;
 
 typedef struct {
     uint64_t firings;          /* per simulation, as in ctl_stats */
     uint64_t ssa_steps;
     uint64_t poisson;
+    uint64_t gaps;
     double   t_end;            /* final clock (t_max + 0.1 after absorption, T.cu:1693) */
     uint32_t leaps, rejections, switches;
     uint32_t events;           /* Philox events consumed */

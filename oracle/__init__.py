@@ -49,7 +49,7 @@ class orc_job(C.Structure):
 class orc_stat(C.Structure):
     _fields_ = [("firings", C.c_uint64), ("leaps", C.c_uint64), ("rejections", C.c_uint64),
                 ("switches", C.c_uint64), ("ssa_steps", C.c_uint64), ("poisson", C.c_uint64),
-                ("t_end", C.c_double), ("events", C.c_uint32), ("absorbed", C.c_uint32)]
+                ("gaps", C.c_uint64), ("t_end", C.c_double), ("events", C.c_uint32), ("absorbed", C.c_uint32)]
 
 
 class orc_model(C.Structure):
@@ -59,7 +59,7 @@ class orc_model(C.Structure):
 
 
 STAT_DTYPE = np.dtype([("firings", "u8"), ("leaps", "u8"), ("rejections", "u8"), ("switches", "u8"),
-                       ("ssa_steps", "u8"), ("poisson", "u8"), ("t_end", "f8"), ("events", "u4"),
+                       ("ssa_steps", "u8"), ("poisson", "u8"), ("gaps", "u8"), ("t_end", "f8"), ("events", "u4"),
                        ("absorbed", "u4")], align=True)
 
 _lib = None
@@ -86,6 +86,8 @@ def lib():
         L.orc_logf.restype = C.c_float
         L.orc_expf.argtypes = [C.c_float]
         L.orc_expf.restype = C.c_float
+        L.orc_log1m.argtypes = [C.c_float]
+        L.orc_log1m.restype = C.c_float
         L.orc_poisson_stream.argtypes = [C.c_uint32] * 4 + [C.c_float, C.c_int, C.c_void_p]
         L.orc_uniform_stream.argtypes = [C.c_uint32] * 4 + [C.c_int, C.c_int, C.c_void_p]
         _lib = L
@@ -171,6 +173,10 @@ def logf(x: float) -> float:
 
 def expf(x: float) -> float:
     return lib().orc_expf(x)
+
+
+def log1m(v: float) -> float:
+    return lib().orc_log1m(v)
 
 
 def poisson_stream(lam: float, count: int, seed=7, sim=0, ev=0) -> np.ndarray:

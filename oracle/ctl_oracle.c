@@ -120,6 +120,31 @@ float orc_expf(float x) {
     return p * u2f((uint32_t)(k + 127) << 23);
 }
 
+/* log(1 - v) for v in [0, 1): the log series above with f = -v exactly while
+ * 1 - v > sqrt(1/2), so small arguments keep full relative precision. */
+float orc_log1m(float v) {
+    if (v < 0x1.2bec34p-2f) {
+        float f = -v;
+        float s = f / (2.0f + f);
+        float z = s * s;
+        float R = fmaf(z, 0x1.c71c72p-3f, 0x1.24924ap-2f);
+        R = fmaf(z, R, 0x1.99999ap-2f);
+        R = fmaf(z, R, 0x1.555556p-1f);
+        R = R * z;
+        float hfsq = 0.5f * f * f;
+        return f - (hfsq - s * (hfsq + R));
+    }
+    return orc_logf(1.0f - v);
+}
+
+/* Exponential(1) variate -log(1 - V): V = r * 2^-32 below 2^-8 (exact) and
+ * (r >> 8) * 2^-24 above, so short gaps keep 32-bit resolution and V < 1. */
+static inline float exp_rv(orc_rng* g) {
+    uint32_t r = rng_next(g);
+    float v = r < 0x01000000u ? (float)r * 0x1p-32f : (float)(r >> 8) * 0x1p-24f;
+    return -orc_log1m(v);
+}
+
 /* ------------------------------------------------------------------ */
 /* Poisson sampler (replaces curand_poisson, T.cu:2018, 2040).           */
 /*  lam < 10        : inversion by sequential search from 0, one U[0,1);  */
@@ -399,23 +424,46 @@ static void ssa_samples(simctx* s, double tn) {
 }
 
 /* Incremental update + negativity test over the fear flat in reaction-major
- * order, test_update T.cu:1300-1351 (active branch T.cu:1320-1331).  Draws
- * are made reaction by reaction from the attempt's event stream; the first
- * negative partial sum rejects (the remaining draws are not needed: a
- * retry opens a new event).  Overflow past INT32_MAX also rejects. */
+ * order, test_update T.cu:1300-1351 (active branch T.cu:1320-1331).  Counts
+ * are drawn reaction by reaction from the attempt's event stream; the first
+ * negative partial sum rejects (the remaining draws are not needed: a retry
+ * opens a new event).  Overflow past INT32_MAX also rejects.
+ *
+ * REFSEM draws an independent Poisson(tau a_j) for every non-critical
+ * reaction, like the reference (curand_poisson, T.cu:2011-2023).  REPLAY
+ * (the GPU algorithm) draws independent Poisson(lambda_j) only for
+ * lambda_j >= 1; the reactions with lambda_j < 1 share one rate-1 Poisson
+ * process whose events are walked lazily in reaction order: reaction j owns
+ * the interval [L_{j-1}, L_j) of the running sum L of their lambdas and fires
+ * once per event in it.  Counts in disjoint intervals are independent
+ * Poisson(lambda_j), so the law is the same with far fewer draws.  The
+ * running sum and the event positions are kept in double. */
 static int apply_leap(simctx* s, int first, int pick, float tauf, double taud, int use_double,
                       uint64_t* fired) {
     const orc_model* m = s->m;
     memcpy(s->X1, s->X, sizeof(int32_t) * m->S);
     *fired = 0;
+    double cum = 0.0, pos = -1.0;
     for (int j = 0; j < m->R; ++j) {
         int32_t k;
-        if (!s->C[j]) {
-            float lam = use_double ? (float)(taud * s->Ad[j]) : tauf * s->Af[j];
-            k = poisson(&s->g, lam);
+        if (s->C[j]) {
+            k = (first && j == pick) ? 1 : 0;
+        } else if (use_double) {
+            k = poisson(&s->g, (float)(taud * s->Ad[j]));
             s->st.poisson++;
         } else {
-            k = (first && j == pick) ? 1 : 0;
+            float lam = tauf * s->Af[j];
+            if (lam < 1.0f) {
+                k = 0;
+                if (lam > 0.0f) {
+                    if (pos < 0.0) { pos = (double)exp_rv(&s->g); s->st.gaps++; }
+                    cum = cum + (double)lam;
+                    while (pos < cum) { ++k; pos = pos + (double)exp_rv(&s->g); s->st.gaps++; }
+                }
+            } else {
+                k = poisson(&s->g, lam);
+                s->st.poisson++;
+            }
         }
         if (k == 0) continue;
         *fired += (uint64_t)k;
@@ -517,7 +565,7 @@ static int leap_f32(simctx* s) {
     if (tau1 < J->ssa_factor / a0) { s->mode = MODE_SSA; s->st.switches++; return LEAP_SWITCH; }  /* T.cu:1935 */
     rng_begin(&s->g, s->evc++);
     float tau2 = FLT_MAX;
-    if (a0c > 0.0f) tau2 = -orc_logf(u01_oc(&s->g)) / a0c;             /* T.cu:1962-1965 */
+    if (a0c > 0.0f) tau2 = exp_rv(&s->g) / a0c;                         /* T.cu:1962-1965 */
     float tau = tau1 < tau2 ? tau1 : tau2;
     int pick = -1;
     if (tau1 > tau2) {                                                 /* T.cu:1993-2006 */
@@ -551,11 +599,10 @@ static void ssa_f32(simctx* s) {
         float a0 = prop_f32(s, NULL);
         if (a0 == 0.0f) { absorb(s); return; }
         float r1 = u01_oc(&s->g) * a0;
-        float r2 = u01_oc(&s->g);
         int j = 0;
         float acc = s->Af[0];
         while (r1 > acc && j < m->R - 1) { ++j; acc = acc + s->Af[j]; }
-        float dt = -orc_logf(r2) / a0;
+        float dt = exp_rv(&s->g) / a0;
         tacc = tacc + (double)dt;
         double tn = s->t + tacc;
         ssa_samples(s, tn);

@@ -61,7 +61,8 @@ typedef struct {
     uint64_t rejections;  /* tau-halving retries */
     uint64_t switches;    /* leap attempts that fell back to SSA */
     uint64_t ssa_steps;   /* SSA steps executed (including overshoot) */
-    uint64_t poisson;     /* Poisson draws made */
+    uint64_t poisson;     /* individual Poisson draws made */
+    uint64_t gaps;        /* exponential gaps of the small-lambda Poisson process (REPLAY) */
     double   t_end;
     uint32_t events;      /* Philox events consumed */
     uint32_t absorbed;
@@ -89,6 +90,7 @@ int orc_run(const orc_net* net, const orc_job* job, uint32_t* storage, orc_stat*
 void  orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float orc_logf(float x);
 float orc_expf(float x);
+float orc_log1m(float v);
 /* count draws from one event stream (key, sim, ev), consumed sequentially. */
 void orc_poisson_stream(uint32_t seed_lo, uint32_t seed_hi, uint32_t sim, uint32_t ev,
                         float lam, int count, int32_t* out);

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 SCH = dict(left=[[1, 2, 0], [0, 3, 0], [0, 0, 1], [0, 1, 0]], right=[[0, 3, 0], [1, 2, 0], [0, 1, 0], [0, 0, 1]],
            x0=[100000, 250, 200000], c=[3e-7, 1e-4, 1e-3, 3.5], feed=[1, 0, 1])
 SCH_TS = [round(0.1 * i, 10) for i in range(1, 101)]
-STAT_FIELDS = ("firings", "leaps", "rejections", "switches", "ssa_steps", "poisson", "events", "absorbed")
+STAT_FIELDS = ("firings", "leaps", "rejections", "switches", "ssa_steps", "poisson", "gaps", "events", "absorbed")
 
 
 def _compare(gpu_sim, ref_out, ref_st, ref_xf):
@@ -200,7 +200,8 @@ def test_cli_end_to_end(gpu, oracle_lib, tmp_path):
     p3 = subprocess.run([gpu.CLI_PATH, str(fold), "32", "2", "0", "0", str(tmp_path / "o2"), "fit", "2", "0"],
                         env=env, capture_output=True, text=True, timeout=300)
     assert p3.returncode == 0, p3.stdout + p3.stderr
-    vals = [float(v) for v in p3.stdout.split("\t") if v.strip()]
+    body = "".join(l for l in p3.stdout.splitlines(True) if not l.startswith("WARNING"))
+    vals = [float(v) for v in body.split("\t") if v.strip()]
     assert len(vals) == 32
     # expected: mean over samples / species of |tgt - mean of the 2 repetitions| / tgt
     exp = []

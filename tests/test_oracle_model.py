@@ -197,3 +197,26 @@ def test_feed_locked_and_nonnegative_random_network(oracle_lib):
     for s in np.nonzero(feed)[0]:
         assert np.all(out[:, :, s] == net["x0"][s])
     assert out.max() < 2 ** 31
+
+
+def test_small_lambda_process_matches_independent_draws(oracle_lib):
+    """The FP32 replay samples reactions with tau*a_j < 1 through one shared
+    Poisson process; the reference-type restatement draws every reaction
+    independently.  Many slow reactions, two-sample KS on every species and
+    sample time (Holm-corrected)."""
+    rng = np.random.default_rng(17)
+    S, M = 6, 40
+    left = np.zeros((M, S), np.int32)
+    right = np.zeros((M, S), np.int32)
+    for j in range(M):
+        if j % 4:
+            left[j, rng.integers(S)] = 1
+        right[j, rng.integers(S)] = 1
+    c = 10 ** rng.uniform(-3, -0.5, M)
+    x0 = rng.integers(50, 400, S)
+    ts = np.linspace(0.5, 20, 8)
+    a, sa = oracle_lib.run(left, right, x0, c, ts, n=3000, seed=1, precision=0)
+    b, sb = oracle_lib.run(left, right, x0, c, ts, n=3000, seed=2, precision=1)
+    assert sa["gaps"].sum() > 0 and sa["leaps"].sum() > 0
+    ps = [su.ks_2samp_p(a[:, i, s], b[:, i, s]) for i in range(len(ts)) for s in range(S)]
+    assert su.holm_ok(ps), min(ps)
