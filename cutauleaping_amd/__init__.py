@@ -48,7 +48,9 @@ EXPORTS = (
     "ctl_model_structure", "ctl_model_load_fitness", "ctl_model_destroy", "ctl_sim_create",
     "ctl_sim_run", "ctl_sim_samples", "ctl_sim_samples_device", "ctl_sim_stats",
     "ctl_sim_sim_stats", "ctl_sim_final_state", "ctl_write_dynamics", "ctl_fitness",
-    "ctl_sim_destroy", "ctl_last_error", "ctl_version", "ctl_device_count",
+    "ctl_sim_destroy", "ctl_last_error", "ctl_version", "ctl_device_count", "ctl_sim_moments",
+    "ctl_write_samples_binary", "ctl_batch_create", "ctl_batch_run", "ctl_batch_samples",
+    "ctl_batch_sim_stats", "ctl_batch_stats", "ctl_batch_stream_base", "ctl_batch_destroy",
 )
 
 
@@ -134,6 +136,16 @@ def lib():
         L.ctl_sim_final_state.argtypes = [vp, vp, vp]
         L.ctl_write_dynamics.argtypes = [vp, C.c_char_p, C.c_int]
         L.ctl_fitness.argtypes = [vp, vp, C.c_size_t]
+        L.ctl_sim_moments.argtypes = [vp, i32, vp, vp, C.c_size_t]
+        L.ctl_write_samples_binary.argtypes = [vp, C.c_char_p]
+        L.ctl_batch_create.argtypes = [vp, vp, i32, C.POINTER(ctl_opts), C.POINTER(vp)]
+        L.ctl_batch_run.argtypes = [vp]
+        L.ctl_batch_samples.argtypes = [vp, i32, vp, C.c_size_t]
+        L.ctl_batch_sim_stats.argtypes = [vp, i32, vp, C.c_size_t]
+        L.ctl_batch_stats.argtypes = [vp, C.POINTER(ctl_stats)]
+        L.ctl_batch_stream_base.argtypes = [vp, i32, C.POINTER(i64)]
+        L.ctl_batch_destroy.argtypes = [vp]
+        L.ctl_batch_destroy.restype = None
         L.ctl_sim_destroy.argtypes = [vp]
         L.ctl_sim_destroy.restype = None
         L.ctl_last_error.restype = C.c_char_p
@@ -278,9 +290,81 @@ class Sim:
         _check(lib().ctl_fitness(self._h, out.ctypes.data, out.size))
         return out
 
+    def moments(self, group: int):
+        """Per-group mean and standard deviation, [n/group, K, SS] each (on the device)."""
+        g = self.n // group
+        mean = np.zeros((g, self.K, self.SS), np.float64)
+        sd = np.zeros_like(mean)
+        _check(lib().ctl_sim_moments(self._h, group, mean.ctypes.data, sd.ctypes.data, mean.size))
+        return mean, sd
+
+    def write_samples_binary(self, path: str) -> None:
+        _check(lib().ctl_write_samples_binary(self._h, path.encode()))
+
     def close(self) -> None:
         if self._h:
             lib().ctl_sim_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many models of uneven size in one persistent launch (C handle)."""
+
+    def __init__(self, models, replicas, *, gpu=0, offset=0, force_ssa=False, seed=0x5EEDC0DE, nc=10,
+                 eps=0.03, ssa_steps=100, ssa_factor=10.0, block_size=0, slots=0, iters_per_launch=0,
+                 ssa_chunk=0):
+        o = default_opts()
+        o.gpu, o.offset, o.force_ssa, o.seed = gpu, offset, int(bool(force_ssa)), seed
+        o.nc, o.eps, o.ssa_steps, o.ssa_factor = nc, eps, ssa_steps, ssa_factor
+        o.block_size, o.slots, o.iters_per_launch, o.ssa_chunk = block_size, slots, iters_per_launch, ssa_chunk
+        self.models = list(models)
+        self.replicas = np.ascontiguousarray(replicas, dtype=np.int64)
+        self.info = [m.info() for m in self.models]
+        handles = (C.c_void_p * len(self.models))(*[m._h.value for m in self.models])
+        self._h = C.c_void_p()
+        _check(lib().ctl_batch_create(handles, self.replicas.ctypes.data, len(self.models), C.byref(o),
+                                      C.byref(self._h)))
+
+    def run(self) -> "Batch":
+        _check(lib().ctl_batch_run(self._h))
+        return self
+
+    def samples(self, model: int) -> np.ndarray:
+        i = self.info[model]
+        out = np.empty(int(self.replicas[model]) * i.K * i.SS, np.uint32)
+        _check(lib().ctl_batch_samples(self._h, model, out.ctypes.data, out.size))
+        return out.reshape(int(self.replicas[model]), i.K, i.SS)
+
+    def sim_stats(self, model: int) -> np.ndarray:
+        out = np.zeros(int(self.replicas[model]), dtype=SIM_STAT_DTYPE)
+        _check(lib().ctl_batch_sim_stats(self._h, model, out.ctypes.data, out.size))
+        return out
+
+    def stream_base(self, model: int) -> int:
+        b = C.c_int64()
+        _check(lib().ctl_batch_stream_base(self._h, model, C.byref(b)))
+        return b.value
+
+    def stats(self) -> dict:
+        s = ctl_stats()
+        _check(lib().ctl_batch_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ctl_stats._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            lib().ctl_batch_destroy(self._h)
             self._h = C.c_void_p()
 
     def __enter__(self):

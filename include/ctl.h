@@ -182,7 +182,40 @@ int ctl_write_dynamics(const ctl_sim* s, const char* path_prefix, int to_stdout)
  * n / p_rep values.  Needs ctl_model_load_fitness. */
 int ctl_fitness(const ctl_sim* s, double* out, size_t count);
 
+/* Scalable output (no counterpart in the reference, whose only output is
+ * one text file per simulation, T.cu:3431-3469).  Per-group moments of the
+ * samples, reduced on the device: simulations [g*group, (g+1)*group) form
+ * group g; mean and population standard deviation per (group, sample,
+ * sampled species), [g][i][sk] doubles, count = (n/group)*K*SS.  n must be a
+ * multiple of group.  Either output pointer may be NULL. */
+int ctl_sim_moments(const ctl_sim* s, int32_t group, double* mean, double* sd, size_t count);
+
+/* Binary dump of the samples: a 32-byte header (magic "CTLS", version 1,
+ * n, K, SS as uint32, 12 reserved bytes) then n*K*SS uint32 in
+ * [simulation][sample][sampled species] order. */
+int ctl_write_samples_binary(const ctl_sim* s, const char* path);
+
 void ctl_sim_destroy(ctl_sim* s);
+
+/* ---- batched models ------------------------------------------------------ */
+/* Many independent models of uneven size and replica count in one persistent
+ * launch.  No counterpart in the reference, which runs one model per process
+ * (a batch replaces N invocations of kernel.cu's main with the same argv
+ * except the input folder).  Model i runs replicas[i] simulations; its
+ * Philox stream indices are offset + base_i + replica, base_i being the sum
+ * of the replica counts of the models before it (ctl_batch_stream_base).
+ * threads / blocks of the options are ignored; every other field applies to
+ * all models.  The models must outlive nothing: they are copied. */
+typedef struct ctl_batch ctl_batch;
+int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, int32_t nmodels,
+                     const ctl_opts* o, ctl_batch** out);
+int ctl_batch_run(ctl_batch* b);
+/* Samples of one model, [replica][sample][sampled species]. */
+int ctl_batch_samples(const ctl_batch* b, int32_t model, uint32_t* out, size_t count);
+int ctl_batch_sim_stats(const ctl_batch* b, int32_t model, ctl_sim_stat* out, size_t count);
+int ctl_batch_stats(const ctl_batch* b, ctl_stats* out);
+int ctl_batch_stream_base(const ctl_batch* b, int32_t model, int64_t* base);
+void ctl_batch_destroy(ctl_batch* b);
 
 /* ---- misc -------------------------------------------------------------- */
 const char* ctl_last_error(void);

@@ -210,3 +210,74 @@ def test_cli_end_to_end(gpu, oracle_lib, tmp_path):
         tgt = np.array([100000, 300, 200000], float)
         exp.append(np.sum(np.abs(tgt - m2) / tgt) / (100 * 3))
     assert np.allclose(vals, exp, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize("force", [False, True])
+def test_enzyme_cascade_bit_exact(gpu, oracle_lib, force):
+    """Config 4's network: a 20-stage enzyme cascade (61 species, 60
+    reactions, rates over six decades), adaptive and force-SSA, 128
+    simulations to t_end=100 with 100 samples."""
+    from cutauleaping_amd import networks
+    print(f"case: enzyme cascade force_ssa={force}", flush=True)
+    net = networks.enzyme_cascade(20, seed=4)
+    ts = networks.sample_times(100, 100.0)
+    m = gpu.Model.from_arrays(net["left"], net["right"], net["x0"], net["c"], ts)
+    with gpu.Sim(m, 128, 1, seed=31, force_ssa=force) as s:
+        s.run()
+        out, st, xf = oracle_lib.run(net["left"], net["right"], net["x0"], net["c"], ts, n=128, seed=31,
+                                     force_ssa=force, final=True)
+        _compare(s, out, st, xf)
+    # conservation per stage: E_k + C_k is constant
+    n = 20
+    assert np.all(xf[:, :n] + xf[:, 2 * n + 1:] == net["x0"][:n] + net["x0"][2 * n + 1:])
+
+
+def test_moments_and_binary_output(gpu, tmp_path):
+    """On-device per-group mean / sd equal numpy's over the same samples; the
+    binary dump round-trips the [sim][sample][species] storage."""
+    print("case: moments + binary output", flush=True)
+    m = gpu.Model.from_arrays(SCH["left"], SCH["right"], SCH["x0"], SCH["c"], SCH_TS, feed=SCH["feed"])
+    with gpu.Sim(m, 64, 8, seed=9) as s:
+        s.run()
+        g = s.samples().astype(np.float64)
+        mean, sd = s.moments(32)
+        path = str(tmp_path / "samples.bin")
+        s.write_samples_binary(path)
+    ref = g.reshape(16, 32, 100, 3)
+    assert np.allclose(mean, ref.mean(axis=1), rtol=1e-12, atol=1e-9)
+    assert np.allclose(sd, ref.std(axis=1), rtol=1e-9, atol=1e-6)
+    raw = np.fromfile(path, dtype=np.uint32)
+    assert raw[0] == 0x534C5443 and raw[1] == 1 and tuple(raw[2:5]) == (512, 100, 3)
+    assert np.array_equal(raw[8:].reshape(512, 100, 3), g.astype(np.uint32))
+
+
+def test_batched_uneven_models_bit_exact(gpu, oracle_lib):
+    """Twelve models of uneven size (S 3..40, R 4..80) and replica count
+    (1..300) in one persistent launch; every model's samples and counters
+    equal the oracle run with that model's stream base.  A small iteration
+    budget forces relaunches mid-simulation."""
+    from cutauleaping_amd import networks
+    print("case: batched uneven models", flush=True)
+    rng = np.random.default_rng(77)
+    models, reps, nets = [], [], []
+    for i in range(12):
+        N = int(rng.integers(3, 41)); M = int(rng.integers(4, 81))
+        net = networks.mass_action(N, M, seed=100 + i, t_end=1.0, check=False)
+        net["x0"] = np.minimum(net["x0"], 3000).astype(np.uint32)
+        ts = np.linspace(0.02, 1.0, int(rng.integers(5, 40)))
+        cs = np.arange(min(N, 7), dtype=np.int32)
+        models.append(gpu.Model.from_arrays(net["left"], net["right"], net["x0"], net["c"], ts, cs=cs))
+        nets.append((net, ts, cs))
+        reps.append(int(rng.integers(1, 301)))
+    with gpu.Batch(models, reps, seed=4242, offset=5, iters_per_launch=50, ssa_chunk=4) as b:
+        b.run()
+        for i, (net, ts, cs) in enumerate(nets):
+            base = b.stream_base(i)
+            out, st = oracle_lib.run(net["left"], net["right"], net["x0"], net["c"], ts, cs=cs, n=reps[i],
+                                     seed=4242, sim_base=5 + base)
+            g = b.samples(i)
+            assert np.array_equal(g, out), i
+            gs = b.sim_stats(i)
+            for f in STAT_FIELDS:
+                assert np.array_equal(gs[f].astype(np.int64), st[f].astype(np.int64)), (i, f)
+        assert b.stats()["simulations"] == sum(reps)
