@@ -262,8 +262,9 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
     for (int i = 0; i < nmodels; ++i) {
         if (!models[i] || replicas[i] <= 0) return ctl::fail(CTL_E_ARGS, "null model or non-positive replica count");
         const ctl::Model& m = models[i]->m;
-        if ((m.x0_rows != 1 && m.x0_rows < replicas[i]) || (m.c_rows != 1 && m.c_rows < replicas[i]))
-            return ctl::fail(CTL_E_ROWS, "a model has fewer input rows than replicas");
+        if ((m.x0_rows > 1 && m.x0_rows < replicas[i] && replicas[i] % m.x0_rows) ||
+            (m.c_rows > 1 && m.c_rows < replicas[i] && replicas[i] % m.c_rows))
+            return ctl::fail(CTL_E_ROWS, "a model's input rows must be 1, at least its replicas, or divide them");
         maxS = std::max(maxS, m.S);
         maxR = std::max(maxR, m.R);
         any_rates = any_rates || m.c_rows != 1;
@@ -322,6 +323,8 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
         KParams p = shared;
         if ((rc = upload_model(b, m, p))) return bail(rc);
         p.n = (int)replicas[i];
+        p.x0_div = (m.x0_rows <= 1 || m.x0_rows >= p.n) ? 1 : (int)(p.n / m.x0_rows);
+        p.c_div = (m.c_rows <= 1 || m.c_rows >= p.n) ? 1 : (int)(p.n / m.c_rows);
         b->base[i] = acc;
         p.sim_base = (uint32_t)(o->offset + acc);
         acc += replicas[i];

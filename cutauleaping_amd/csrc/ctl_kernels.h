@@ -64,9 +64,10 @@ struct KParams {
     const double* ts;
     double tmax;
     const float* c_bcast;     // R (rates shared by every simulation) or null
-    const float* c_sims;      // [sim][R] or null
+    const float* c_sims;      // [row][R] or null; row = sim / c_div
     const uint32_t* x0_bcast; // S or null
-    const uint32_t* x0_sims;  // [sim][S] or null
+    const uint32_t* x0_sims;  // [row][S] or null; row = sim / x0_div
+    int c_div, x0_div;        // simulations per input row (1: one row each)
     // run
     int n;
     uint32_t sim_base, key0, key1;
@@ -473,7 +474,7 @@ __device__ __forceinline__ void store_lane(const KParams& p, const Lane& L) {
 // Per-simulation rates into the lane's rate array (rows that differ).
 __device__ __forceinline__ void load_rates(const KParams& p, const View& V, int sim) {
     if (!p.c_bcast)
-        for (int j = 0; j < p.R; ++j) V.Cs[(size_t)j * V.st] = p.c_sims[(size_t)sim * p.R + j];
+        for (int j = 0; j < p.R; ++j) V.Cs[(size_t)j * V.st] = p.c_sims[(size_t)(sim / p.c_div) * p.R + j];
 }
 
 // Takes the next simulation from the queue and initialises the lane
@@ -494,7 +495,7 @@ __device__ __forceinline__ void start(const KParams& p, Lane& L, const View& V, 
     L.g.pos = 4;
     int32_t* X = xcur(V, L);
     for (int s = 0; s < p.S; ++s)
-        X[(size_t)s * V.st] = (int32_t)(p.x0_bcast ? CT_U32(p.x0_bcast)[s] : p.x0_sims[(size_t)q * p.S + s]);
+        X[(size_t)s * V.st] = (int32_t)(p.x0_bcast ? CT_U32(p.x0_bcast)[s] : p.x0_sims[(size_t)(q / (uint32_t)p.x0_div) * p.S + s]);
     load_rates(p, V, (int)q);
 }
 

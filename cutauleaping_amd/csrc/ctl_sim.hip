@@ -268,8 +268,12 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     const int64_t n64 = (int64_t)o->threads * o->blocks;
     if (n64 > 0x7fffffffLL) return ctl::fail(CTL_E_LIMIT, "too many simulations");
     const int n = (int)n64;
-    if (m.x0_rows != 1 && m.x0_rows < n) return ctl::fail(CTL_E_ROWS, "MX_0 has fewer rows than simulations");
-    if (m.c_rows != 1 && m.c_rows < n) return ctl::fail(CTL_E_ROWS, "c_matrix has fewer rows than simulations");
+    // rows: 1 = shared, >= n = one per simulation, otherwise n/rows
+    // consecutive simulations (a replica group) per row
+    if (m.x0_rows > 1 && m.x0_rows < n && n % m.x0_rows)
+        return ctl::fail(CTL_E_ROWS, "MX_0 rows must be 1, at least the simulation count, or divide it");
+    if (m.c_rows > 1 && m.c_rows < n && n % m.c_rows)
+        return ctl::fail(CTL_E_ROWS, "c_matrix rows must be 1, at least the simulation count, or divide it");
     if (o->nc < 0 || o->ssa_steps <= 0 || !(o->eps > 0.0f)) return ctl::fail(CTL_E_ARGS, "bad tunables");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -363,6 +367,8 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     p.c_sims = m.c_rows == 1 ? nullptr : cdev;
     p.x0_bcast = m.x0_rows == 1 ? xdev : nullptr;
     p.x0_sims = m.x0_rows == 1 ? nullptr : xdev;
+    p.x0_div = (m.x0_rows <= 1 || m.x0_rows >= n) ? 1 : (int)(n / m.x0_rows);
+    p.c_div = (m.c_rows <= 1 || m.c_rows >= n) ? 1 : (int)(n / m.c_rows);
     p.n = n;
     p.sim_base = (uint32_t)o->offset;
     p.key0 = (uint32_t)o->seed;

@@ -52,9 +52,11 @@ typedef struct {
     const int32_t* right;      /* R*S row-major [r][s], >= 0 (right_side) */
     const int32_t* feed;       /* S flags, non-zero = feed-locked (M_feed); NULL = none */
     const uint32_t* x0;        /* x0_rows * S initial counts (MX_0 rows) */
-    int64_t x0_rows;           /* 1 = the row is shared by every simulation */
+    int64_t x0_rows;           /* 1: shared by every simulation; >= n: one row per
+                                  simulation (the reference); otherwise n/x0_rows
+                                  consecutive simulations share a row (replicas) */
     const double* c;           /* c_rows * R rate constants (c_matrix rows) */
-    int64_t c_rows;            /* 1 = the row is shared by every simulation */
+    int64_t c_rows;            /* same rule as x0_rows */
     const double* t_vector;    /* K sample times, t_max = t_vector[K-1] */
     int32_t K;
     const int32_t* cs_vector;  /* SS sampled species; NULL = all species */

@@ -765,9 +765,21 @@ static void run_one(simctx* s) {
     s->st.events = s->evc;
 }
 
+/* Input row of simulation i: one shared row, one row per simulation
+ * (rows >= n, the reference's MX_0 / c_matrix), or n / rows consecutive
+ * simulations per row (replica groups). */
+static int row_of(int rows, int n, int i) {
+    if (rows <= 1) return 0;
+    if (rows >= n) return i;
+    return i / (n / rows);
+}
+
 int orc_run(const orc_net* net, const orc_job* J, uint32_t* storage, orc_stat* stats, int32_t* xfinal) {
     orc_model m;
     if (J->K <= 0 || J->SS < 0 || J->n < 0) return -1;
+    if ((J->x0_rows > 1 && J->x0_rows < J->n && J->n % J->x0_rows) ||
+        (J->c_rows > 1 && J->c_rows < J->n && J->n % J->c_rows))
+        return -1;
     for (int sk = 0; sk < J->SS; ++sk)
         if (J->cs[sk] < 0 || J->cs[sk] >= net->S) return -1;
     if (orc_compile(net, &m) != 0) return -1;
@@ -789,11 +801,11 @@ int orc_run(const orc_net* net, const orc_job* J, uint32_t* storage, orc_stat* s
             simctx s;
             memset(&s, 0, sizeof(s));
             s.m = &m; s.J = J;
-            s.c = J->c + (size_t)(J->c_rows == 1 ? 0 : i) * R;
+            s.c = J->c + (size_t)row_of(J->c_rows, J->n, i) * R;
             for (int r = 0; r < R; ++r) cf[r] = (float)s.c[r];
             s.cf = cf; s.Af = Af; s.Ad = Ad; s.C = C;
             s.X = X; s.X1 = X1;
-            const uint32_t* x0 = J->x0 + (size_t)(J->x0_rows == 1 ? 0 : i) * S;
+            const uint32_t* x0 = J->x0 + (size_t)row_of(J->x0_rows, J->n, i) * S;
             for (int sp = 0; sp < S; ++sp) X[sp] = (int32_t)x0[sp];
             s.out = storage + (size_t)i * J->K * J->SS;
             memset(s.out, 0, sizeof(uint32_t) * (size_t)J->K * J->SS);   /* PrepareVariables T.cu:1269-1297 */

@@ -40,8 +40,9 @@ typedef struct {
 
 typedef struct {
     int n;                          /* simulations */
-    const uint32_t* x0; int x0_rows;/* [row][s]; x0_rows == 1 broadcasts row 0 */
-    const double* c; int c_rows;    /* [row][r]; c_rows == 1 broadcasts row 0 */
+    const uint32_t* x0; int x0_rows;/* [row][s]; 1 row: shared; >= n rows: one per
+                                       simulation; else n/rows simulations per row */
+    const double* c; int c_rows;    /* [row][r]; same rule */
     const double* ts; int K;        /* sample times, t_max = ts[K-1] */
     const int32_t* cs; int SS;      /* sampled species */
     uint32_t seed_lo, seed_hi;      /* Philox key */

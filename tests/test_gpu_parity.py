@@ -281,3 +281,24 @@ def test_batched_uneven_models_bit_exact(gpu, oracle_lib):
             for f in STAT_FIELDS:
                 assert np.array_equal(gs[f].astype(np.int64), st[f].astype(np.int64)), (i, f)
         assert b.stats()["simulations"] == sum(reps)
+
+
+def test_replica_group_rows_bit_exact(gpu, oracle_lib):
+    """Rate and initial-state rows shared by groups of consecutive
+    simulations (16 parameter sets x 32 replicas), as a parameter sweep uses
+    them; per-group moments come from the device."""
+    from cutauleaping_amd import networks
+    print("case: replica-group rows", flush=True)
+    net = networks.mass_action(10, 20, seed=8, t_end=2.0)
+    rng = np.random.default_rng(8)
+    c = net["c"][None, :] * 10 ** rng.uniform(-1, 1, (16, 20))
+    x0 = (net["x0"][None, :] * rng.uniform(0.5, 1.5, (16, 10))).astype(np.uint32)
+    ts = np.linspace(0.1, 2.0, 20)
+    m = gpu.Model.from_arrays(net["left"], net["right"], x0, c, ts)
+    with gpu.Sim(m, 32, 16, seed=12) as s:
+        s.run()
+        out, st = oracle_lib.run(net["left"], net["right"], x0, c, ts, n=512, seed=12)
+        assert np.array_equal(s.samples(), out)
+        mean, sd = s.moments(32)
+    ref = out.reshape(16, 32, 20, 10).astype(float)
+    assert np.allclose(mean, ref.mean(axis=1)) and np.allclose(sd, ref.std(axis=1), atol=1e-6)
