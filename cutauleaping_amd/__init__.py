@@ -51,6 +51,8 @@ EXPORTS = (
     "ctl_sim_destroy", "ctl_last_error", "ctl_version", "ctl_device_count", "ctl_sim_moments",
     "ctl_write_samples_binary", "ctl_batch_create", "ctl_batch_run", "ctl_batch_samples",
     "ctl_batch_sim_stats", "ctl_batch_stats", "ctl_batch_stream_base", "ctl_batch_destroy",
+    "ctl_mlp_create", "ctl_mlp_num_params", "ctl_mlp_params", "ctl_mlp_forward", "ctl_mlp_forward_device",
+    "ctl_mlp_gradient", "ctl_mlp_set_data", "ctl_mlp_train", "ctl_mlp_destroy",
 )
 
 
@@ -148,6 +150,16 @@ def lib():
         L.ctl_batch_destroy.restype = None
         L.ctl_sim_destroy.argtypes = [vp]
         L.ctl_sim_destroy.restype = None
+        L.ctl_mlp_create.argtypes = [vp, i32, i32, C.c_uint64, C.POINTER(vp)]
+        L.ctl_mlp_num_params.argtypes = [vp, C.POINTER(i64)]
+        L.ctl_mlp_params.argtypes = [vp, vp, C.c_size_t, i32]
+        L.ctl_mlp_forward.argtypes = [vp, vp, i64, vp]
+        L.ctl_mlp_forward_device.argtypes = [vp, vp, i64, vp, C.POINTER(C.c_float)]
+        L.ctl_mlp_gradient.argtypes = [vp, vp, vp, i32, vp, C.c_size_t, C.POINTER(C.c_float)]
+        L.ctl_mlp_set_data.argtypes = [vp, vp, vp, i64]
+        L.ctl_mlp_train.argtypes = [vp, i32, i32, C.c_float, C.c_uint64, vp]
+        L.ctl_mlp_destroy.argtypes = [vp]
+        L.ctl_mlp_destroy.restype = None
         L.ctl_last_error.restype = C.c_char_p
         L.ctl_version.restype = C.c_char_p
         L.ctl_device_count.restype = C.c_int
@@ -365,6 +377,79 @@ class Batch:
     def close(self) -> None:
         if self._h:
             lib().ctl_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MLP:
+    """BF16-MFMA multilayer perceptron on one GPU (C handle): ReLU hidden
+    layers, linear output, FP32 master weights trained with Adam."""
+
+    def __init__(self, widths, *, gpu=0, seed=1):
+        self.widths = [int(w) for w in widths]
+        w = np.ascontiguousarray(self.widths, dtype=np.int32)
+        self._h = C.c_void_p()
+        _check(lib().ctl_mlp_create(w.ctypes.data, len(w) - 1, gpu, seed, C.byref(self._h)))
+        n = C.c_int64()
+        _check(lib().ctl_mlp_num_params(self._h, C.byref(n)))
+        self.nparam = n.value
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.nparam, np.float32)
+        _check(lib().ctl_mlp_params(self._h, out.ctypes.data, out.size, 0))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.nparam:
+            raise ValueError(f"expected {self.nparam} parameters, got {p.size}")
+        _check(lib().ctl_mlp_params(self._h, p.ctypes.data, p.size, 1))
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.widths[0])
+        y = np.empty((x.shape[0], self.widths[-1]), np.float32)
+        _check(lib().ctl_mlp_forward(self._h, x.ctypes.data, x.shape[0], y.ctypes.data))
+        return y
+
+    def forward_device(self, x_ptr: int, rows: int, y_ptr: int) -> float:
+        """Forward pass on device buffers (raw pointers); returns device ms."""
+        ms = C.c_float()
+        _check(lib().ctl_mlp_forward_device(self._h, C.c_void_p(x_ptr), rows, C.c_void_p(y_ptr), C.byref(ms)))
+        return ms.value
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.widths[0])
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(x.shape[0], self.widths[-1])
+        g = np.empty(self.nparam, np.float32)
+        loss = C.c_float()
+        _check(lib().ctl_mlp_gradient(self._h, x.ctypes.data, t.ctypes.data, x.shape[0], g.ctypes.data, g.size,
+                                      C.byref(loss)))
+        return loss.value, g
+
+    def set_data(self, x, t) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.widths[0])
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(x.shape[0], self.widths[-1])
+        _check(lib().ctl_mlp_set_data(self._h, x.ctypes.data, t.ctypes.data, x.shape[0]))
+
+    def train(self, epochs: int, batch: int, lr: float, seed: int = 0) -> np.ndarray:
+        losses = np.empty(epochs, np.float32)
+        _check(lib().ctl_mlp_train(self._h, epochs, batch, lr, seed, losses.ctypes.data))
+        return losses
+
+    def close(self) -> None:
+        if self._h:
+            lib().ctl_mlp_destroy(self._h)
             self._h = C.c_void_p()
 
     def __enter__(self):

@@ -1,0 +1,499 @@
+// ctl_mlp.hip -- the learned surrogate's MLP on gfx950: hand-written BF16
+// MFMA GEMMs with FP32 accumulation for inference and training (forward,
+// backward, Adam).  No counterpart in the reference (the surrogate is an
+// addition of this build); the simulator is its data source, its fallback
+// and the judge of its error.
+//
+// Network: widths w0 -> w1 -> ... -> wL, ReLU after every hidden layer,
+// linear output.  Weights W_l are [w_{l-1}][w_l] row-major; activations are
+// [batch][width] row-major.  Master weights, Adam moments and gradients are
+// FP32; the GEMM operands (weights, activations, back-propagated errors) are
+// BF16 (round to nearest even); every product accumulates in FP32.
+//
+// GEMM: one wavefront computes a 32x32 tile of C with
+// v_mfma_f32_32x32x8_bf16_1k; a 256-lane workgroup covers 64x64.  Operand
+// element (i, k) of A is A[i*ai + k*ak], (k, j) of B is B[k*bk + j*bj], so
+// the same kernel serves X*W (forward), dY*W^T (input error) and X^T*dY
+// (weight gradient).  Operand lane layout of the 32x32x8 MFMA: lane l holds
+// A[l%32][4*(l/32)+0..3] and B[4*(l/32)+0..3][l%32]; accumulator v of lane l
+// is C[8*(v/4) + 4*(l/32) + v%4][l%32].
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "ctl_internal.h"
+
+namespace ctl {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+enum : int { EPI_STORE_F32 = 0, EPI_BIAS_RELU_BF16 = 1, EPI_BIAS_F32 = 2, EPI_MASK_BF16 = 3 };
+
+struct GemmArgs {
+    const uint16_t* A;
+    const uint16_t* B;
+    long ai, ak, bk, bj;
+    int M, N, K;
+    int epi;
+    const float* bias;         // [N] (EPI_BIAS_*)
+    const uint16_t* mask;      // [M][N] bf16 activation whose sign gates (EPI_MASK_BF16)
+    float* Cf;                 // [M][N] fp32 out
+    uint16_t* Cb;              // [M][N] bf16 out
+};
+
+__global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int row0 = blockIdx.y * 64 + (wave >> 1) * 32;
+    const int col0 = blockIdx.x * 64 + (wave & 1) * 32;
+    const int r = row0 + (lane & 31);
+    const int c = col0 + (lane & 31);
+    const int kq = (lane >> 5) * 4;
+    f32x16 acc;
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+    for (int k0 = 0; k0 < g.K; k0 += 8) {
+        bf16x4 a, b;
+        for (int q = 0; q < 4; ++q) {
+            const int k = k0 + kq + q;
+            a[q] = (r < g.M && k < g.K) ? (short)g.A[(long)r * g.ai + (long)k * g.ak] : (short)0;
+            b[q] = (c < g.N && k < g.K) ? (short)g.B[(long)k * g.bk + (long)c * g.bj] : (short)0;
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    for (int v = 0; v < 16; ++v) {
+        const int row = row0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (row >= g.M || c >= g.N) continue;
+        const long o = (long)row * g.N + c;
+        float x = acc[v];
+        switch (g.epi) {
+            case EPI_STORE_F32: g.Cf[o] = x; break;
+            case EPI_BIAS_RELU_BF16: x += g.bias[c]; g.Cb[o] = f2bf(x > 0.0f ? x : 0.0f); break;
+            case EPI_BIAS_F32: g.Cf[o] = x + g.bias[c]; break;
+            case EPI_MASK_BF16: g.Cb[o] = f2bf(bf2f(g.mask[o]) > 0.0f ? x : 0.0f); break;
+        }
+    }
+}
+
+// Loss gradient of the mean squared error over [B][N]: d = 2 (y - t) / (B N),
+// stored as BF16 for the backward GEMMs; per-block partial losses in FP32.
+__global__ void mse_grad(const float* __restrict__ y, const float* __restrict__ t, uint16_t* __restrict__ d,
+                         float* __restrict__ dscale_out, float* __restrict__ loss_part, long total, float scale) {
+    __shared__ float red[256];
+    float acc = 0.0f;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const float e = y[i] - t[i];
+        acc += e * e;
+        d[i] = f2bf(2.0f * e * scale);
+        if (dscale_out) dscale_out[i] = 2.0f * e * scale;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+}
+
+// db[j] = sum_b d[b][j] (d in BF16), FP32.
+__global__ void col_sum(const uint16_t* __restrict__ d, float* __restrict__ db, int B, int N) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    float s = 0.0f;
+    for (int b = 0; b < B; ++b) s += bf2f(d[(long)b * N + j]);
+    db[j] = s;
+}
+
+__global__ void to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = f2bf(x[i]);
+}
+
+// Rows idx[0..B) of a [n][w] FP32 array, as BF16 (inputs) or FP32 (targets).
+__global__ void gather_rows(const float* __restrict__ src, const int32_t* __restrict__ idx, uint16_t* __restrict__ dstb,
+                            float* __restrict__ dstf, int B, int w) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)B * w) return;
+    const long b = i / w, k = i % w;
+    const float v = src[(long)idx[b] * w + k];
+    if (dstb) dstb[i] = f2bf(v);
+    if (dstf) dstf[i] = v;
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master copy, refreshing the BF16 copy.
+__global__ void adam(float* __restrict__ w, uint16_t* __restrict__ wb, const float* __restrict__ gr,
+                     float* __restrict__ m, float* __restrict__ v, long n, float lr, float b1, float b2, float eps,
+                     float c1, float c2) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = gr[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * g;
+    const float vi = b2 * v[i] + (1.0f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = f2bf(wi);
+}
+
+}  // namespace ctl
+
+using namespace ctl;
+
+struct ctl_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<int> widths;
+    int L = 0;                          // layers
+    std::vector<long> woff, boff;       // offsets into the flat parameter vector
+    long nparam = 0;
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;   // flat fp32: W_1, b_1, ..., W_L, b_L
+    uint16_t* wb = nullptr;             // bf16 copy (biases unused)
+    long step = 0;
+    // work buffers for a batch of up to cap rows
+    int cap = 0;
+    std::vector<uint16_t*> act;         // act[l]: [cap][w_l] bf16, l = 0..L-1
+    uint16_t *err0 = nullptr, *err1 = nullptr;   // [cap][max w] bf16
+    float* yout = nullptr;              // [cap][w_L]
+    float* tgt = nullptr;
+    float* part = nullptr;
+    int32_t* idx = nullptr;
+    // training data
+    float *dX = nullptr, *dY = nullptr;
+    long ndata = 0;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int mfail(hipError_t e, const char* what) {
+    char b[256];
+    std::snprintf(b, sizeof b, "%s: %s", what, hipGetErrorString(e));
+    return ctl::fail(e == hipErrorOutOfMemory ? CTL_E_NOMEM : CTL_E_DEVICE, b);
+}
+
+#define MHIP(call, what)                             \
+    do {                                             \
+        hipError_t e_ = (call);                      \
+        if (e_ != hipSuccess) return mfail(e_, what); \
+    } while (0)
+
+template <typename T>
+int malloc_dev(ctl_mlp* n, T** p, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) return mfail(e, "hipMalloc");
+    n->allocs.push_back(q);
+    *p = (T*)q;
+    return 0;
+}
+
+void gemm(ctl_mlp* n, const uint16_t* A, long ai, long ak, const uint16_t* B, long bk, long bj, int M, int N, int K,
+          int epi, const float* bias, const uint16_t* mask, float* Cf, uint16_t* Cb) {
+    GemmArgs g{A, B, ai, ak, bk, bj, M, N, K, epi, bias, mask, Cf, Cb};
+    dim3 grid((N + 63) / 64, (M + 63) / 64);
+    hipLaunchKernelGGL(gemm_bf16, grid, dim3(256), 0, n->stream, g);
+}
+
+int ensure_cap(ctl_mlp* n, int B) {
+    if (B <= n->cap) return 0;
+    int wmax = 0;
+    for (int w : n->widths) wmax = std::max(wmax, w);
+    n->act.assign(n->L, nullptr);
+    int rc;
+    for (int l = 0; l < n->L; ++l)
+        if ((rc = malloc_dev(n, &n->act[l], (size_t)B * n->widths[l]))) return rc;
+    if ((rc = malloc_dev(n, &n->err0, (size_t)B * wmax)) || (rc = malloc_dev(n, &n->err1, (size_t)B * wmax)) ||
+        (rc = malloc_dev(n, &n->yout, (size_t)B * n->widths[n->L])) ||
+        (rc = malloc_dev(n, &n->tgt, (size_t)B * n->widths[n->L])) || (rc = malloc_dev(n, &n->part, 1024)) ||
+        (rc = malloc_dev(n, &n->idx, B)))
+        return rc;
+    n->cap = B;
+    return 0;
+}
+
+// Forward pass of B rows already in act[0]; output FP32 in yout.
+void forward(ctl_mlp* n, int B) {
+    for (int l = 0; l < n->L; ++l) {
+        const int K = n->widths[l], N = n->widths[l + 1];
+        const uint16_t* W = n->wb + n->woff[l];
+        const float* b = n->w + n->boff[l];
+        if (l + 1 < n->L)
+            gemm(n, n->act[l], K, 1, W, N, 1, B, N, K, EPI_BIAS_RELU_BF16, b, nullptr, nullptr, n->act[l + 1]);
+        else
+            gemm(n, n->act[l], K, 1, W, N, 1, B, N, K, EPI_BIAS_F32, b, nullptr, n->yout, nullptr);
+    }
+}
+
+// Backward pass from the BF16 output error in err0: fills n->g.
+void backward(ctl_mlp* n, int B) {
+    uint16_t* e = n->err0;
+    uint16_t* e2 = n->err1;
+    for (int l = n->L - 1; l >= 0; --l) {
+        const int K = n->widths[l], N = n->widths[l + 1];
+        // dW_l = act[l]^T * e : (K x B) * (B x N)
+        gemm(n, n->act[l], 1, K, e, N, 1, K, N, B, EPI_STORE_F32, nullptr, nullptr, n->g + n->woff[l], nullptr);
+        hipLaunchKernelGGL(col_sum, dim3((N + 255) / 256), dim3(256), 0, n->stream, e, n->g + n->boff[l], B, N);
+        if (l > 0) {
+            // e_{l} = (e * W_l^T) gated by act[l] > 0 : (B x N) * (N x K)
+            const uint16_t* W = n->wb + n->woff[l];
+            gemm(n, e, N, 1, W, 1, N, B, K, N, EPI_MASK_BF16, nullptr, n->act[l], nullptr, e2);
+            std::swap(e, e2);
+        }
+    }
+}
+
+float loss_and_grad(ctl_mlp* n, int B, const float* t_dev, float* dscale_dbg) {
+    const long total = (long)B * n->widths[n->L];
+    const int blocks = (int)std::min<long>(1024, (total + 255) / 256);
+    hipLaunchKernelGGL(mse_grad, dim3(blocks), dim3(256), 0, n->stream, n->yout, t_dev, n->err0, dscale_dbg, n->part,
+                       total, 1.0f / (float)total);
+    std::vector<float> part(blocks);
+    if (hipMemcpyAsync(part.data(), n->part, sizeof(float) * blocks, hipMemcpyDeviceToHost, n->stream) != hipSuccess)
+        return NAN;
+    if (hipStreamSynchronize(n->stream) != hipSuccess) return NAN;
+    double s = 0.0;
+    for (float p : part) s += p;
+    return (float)(s / (double)total);
+}
+
+void refresh_bf16(ctl_mlp* n) {
+    hipLaunchKernelGGL(to_bf16, dim3((unsigned)((n->nparam + 255) / 256)), dim3(256), 0, n->stream, n->w, n->wb,
+                       n->nparam);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ctl_mlp_create(const int32_t* widths, int32_t nlayers, int32_t gpu, uint64_t seed, ctl_mlp** out) {
+    ctl::clear_error();
+    if (!widths || nlayers < 1 || !out) return ctl::fail(CTL_E_ARGS, "bad arguments");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return ctl::fail(CTL_E_DEVICE, "no HIP device available: libctl needs an AMD GPU (gfx950)");
+    for (int l = 0; l <= nlayers; ++l)
+        if (widths[l] <= 0) return ctl::fail(CTL_E_ARGS, "layer widths must be positive");
+    ctl_mlp* n = new ctl_mlp();
+    auto bail = [&](int rc) { ctl_mlp_destroy(n); return rc; };
+    n->device = gpu;
+    n->L = nlayers;
+    n->widths.assign(widths, widths + nlayers + 1);
+    hipError_t e = hipSetDevice(gpu);
+    if (e != hipSuccess) return bail(mfail(e, "hipSetDevice"));
+    if ((e = hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking)) != hipSuccess) return bail(mfail(e, "stream"));
+    long off = 0;
+    for (int l = 0; l < nlayers; ++l) {
+        n->woff.push_back(off);
+        off += (long)widths[l] * widths[l + 1];
+        n->boff.push_back(off);
+        off += widths[l + 1];
+    }
+    n->nparam = off;
+    int rc;
+    if ((rc = malloc_dev(n, &n->w, off)) || (rc = malloc_dev(n, &n->g, off)) || (rc = malloc_dev(n, &n->m, off)) ||
+        (rc = malloc_dev(n, &n->v, off)) || (rc = malloc_dev(n, &n->wb, off)))
+        return bail(rc);
+    // He-uniform weights, zero biases (host RNG, seeded)
+    std::vector<float> h(off, 0.0f);
+    std::mt19937_64 rng(seed);
+    for (int l = 0; l < nlayers; ++l) {
+        const float a = std::sqrt(6.0f / (float)widths[l]);
+        std::uniform_real_distribution<float> U(-a, a);
+        for (long i = 0; i < (long)widths[l] * widths[l + 1]; ++i) h[n->woff[l] + i] = U(rng);
+    }
+    if ((e = hipMemcpy(n->w, h.data(), sizeof(float) * off, hipMemcpyHostToDevice)) != hipSuccess) return bail(mfail(e, "H2D"));
+    if ((e = hipMemset(n->m, 0, sizeof(float) * off)) != hipSuccess || (e = hipMemset(n->v, 0, sizeof(float) * off)) != hipSuccess)
+        return bail(mfail(e, "memset"));
+    refresh_bf16(n);
+    if ((e = hipStreamSynchronize(n->stream)) != hipSuccess) return bail(mfail(e, "init"));
+    *out = n;
+    return 0;
+}
+
+int ctl_mlp_num_params(const ctl_mlp* n, int64_t* count) {
+    if (!n || !count) return ctl::fail(CTL_E_ARGS, "null argument");
+    *count = n->nparam;
+    return 0;
+}
+
+int ctl_mlp_params(ctl_mlp* n, float* host, size_t count, int32_t set) {
+    ctl::clear_error();
+    if (!n || !host || count != (size_t)n->nparam) return ctl::fail(CTL_E_ARGS, "count must equal the parameter count");
+    MHIP(hipSetDevice(n->device), "hipSetDevice");
+    if (set) {
+        MHIP(hipMemcpy(n->w, host, sizeof(float) * count, hipMemcpyHostToDevice), "H2D");
+        refresh_bf16(n);
+        MHIP(hipStreamSynchronize(n->stream), "sync");
+    } else {
+        MHIP(hipMemcpy(host, n->w, sizeof(float) * count, hipMemcpyDeviceToHost), "D2H");
+    }
+    return 0;
+}
+
+int ctl_mlp_forward(ctl_mlp* n, const float* x, int64_t rows, float* y) {
+    ctl::clear_error();
+    if (!n || !x || !y || rows <= 0) return ctl::fail(CTL_E_ARGS, "bad arguments");
+    MHIP(hipSetDevice(n->device), "hipSetDevice");
+    const int chunk = 8192;
+    int rc = ensure_cap(n, (int)std::min<int64_t>(rows, chunk));
+    if (rc) return rc;
+    float* xin = nullptr;
+    MHIP(hipMalloc(&xin, sizeof(float) * (size_t)std::min<int64_t>(rows, chunk) * n->widths[0]), "hipMalloc");
+    for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+        const int B = (int)std::min<int64_t>(chunk, rows - r0);
+        const long nin = (long)B * n->widths[0];
+        hipError_t e = hipMemcpyAsync(xin, x + r0 * n->widths[0], sizeof(float) * nin, hipMemcpyHostToDevice, n->stream);
+        if (e != hipSuccess) { (void)hipFree(xin); return mfail(e, "H2D"); }
+        hipLaunchKernelGGL(to_bf16, dim3((unsigned)((nin + 255) / 256)), dim3(256), 0, n->stream, xin, n->act[0], nin);
+        forward(n, B);
+        e = hipMemcpyAsync(y + r0 * n->widths[n->L], n->yout, sizeof(float) * (size_t)B * n->widths[n->L],
+                           hipMemcpyDeviceToHost, n->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+        if (e != hipSuccess) { (void)hipFree(xin); return mfail(e, "forward"); }
+    }
+    (void)hipFree(xin);
+    return 0;
+}
+
+int ctl_mlp_forward_device(ctl_mlp* n, const float* x_dev, int64_t rows, float* y_dev, float* ms) {
+    ctl::clear_error();
+    if (!n || !x_dev || !y_dev || rows <= 0) return ctl::fail(CTL_E_ARGS, "bad arguments");
+    MHIP(hipSetDevice(n->device), "hipSetDevice");
+    const int chunk = 65536;
+    int rc = ensure_cap(n, (int)std::min<int64_t>(rows, chunk));
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ms) {
+        MHIP(hipEventCreate(&e0), "event");
+        MHIP(hipEventCreate(&e1), "event");
+        MHIP(hipEventRecord(e0, n->stream), "event");
+    }
+    const int w0 = n->widths[0], wL = n->widths[n->L];
+    for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+        const int B = (int)std::min<int64_t>(chunk, rows - r0);
+        const long nin = (long)B * w0;
+        hipLaunchKernelGGL(to_bf16, dim3((unsigned)((nin + 255) / 256)), dim3(256), 0, n->stream, x_dev + r0 * w0,
+                           n->act[0], nin);
+        forward(n, B);
+        hipError_t e = hipMemcpyAsync(y_dev + r0 * wL, n->yout, sizeof(float) * (size_t)B * wL,
+                                      hipMemcpyDeviceToDevice, n->stream);
+        if (e != hipSuccess) return mfail(e, "D2D");
+    }
+    hipError_t e = hipSuccess;
+    if (ms) e = hipEventRecord(e1, n->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+    if (e == hipSuccess && ms) e = hipEventElapsedTime(ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return mfail(e, "forward");
+    return 0;
+}
+
+// One forward + backward on rows (x, t); returns the loss and the FP32
+// gradient of the mean squared error with respect to every parameter.
+int ctl_mlp_gradient(ctl_mlp* n, const float* x, const float* t, int32_t rows, float* grad, size_t count, float* loss) {
+    ctl::clear_error();
+    if (!n || !x || !t || rows <= 0 || rows > 65536 || (grad && count != (size_t)n->nparam)) return ctl::fail(CTL_E_ARGS, "bad arguments");
+    MHIP(hipSetDevice(n->device), "hipSetDevice");
+    int rc = ensure_cap(n, rows);
+    if (rc) return rc;
+    const long nin = (long)rows * n->widths[0], nout = (long)rows * n->widths[n->L];
+    float* xin = nullptr;
+    MHIP(hipMalloc(&xin, sizeof(float) * nin), "hipMalloc");
+    hipError_t e = hipMemcpy(xin, x, sizeof(float) * nin, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(n->tgt, t, sizeof(float) * nout, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { (void)hipFree(xin); return mfail(e, "H2D"); }
+    hipLaunchKernelGGL(to_bf16, dim3((unsigned)((nin + 255) / 256)), dim3(256), 0, n->stream, xin, n->act[0], nin);
+    forward(n, rows);
+    const float l = loss_and_grad(n, rows, n->tgt, nullptr);
+    backward(n, rows);
+    e = hipStreamSynchronize(n->stream);
+    if (e == hipSuccess && grad) e = hipMemcpy(grad, n->g, sizeof(float) * count, hipMemcpyDeviceToHost);
+    (void)hipFree(xin);
+    if (e != hipSuccess) return mfail(e, "gradient");
+    if (loss) *loss = l;
+    return 0;
+}
+
+// Training data (normalised inputs [n][w0], targets [n][wL]) kept on the device.
+int ctl_mlp_set_data(ctl_mlp* n, const float* x, const float* t, int64_t rows) {
+    ctl::clear_error();
+    if (!n || !x || !t || rows <= 0) return ctl::fail(CTL_E_ARGS, "bad arguments");
+    MHIP(hipSetDevice(n->device), "hipSetDevice");
+    if (n->dX) { (void)hipFree(n->dX); n->dX = nullptr; }
+    if (n->dY) { (void)hipFree(n->dY); n->dY = nullptr; }
+    MHIP(hipMalloc(&n->dX, sizeof(float) * rows * n->widths[0]), "hipMalloc");
+    MHIP(hipMalloc(&n->dY, sizeof(float) * rows * n->widths[n->L]), "hipMalloc");
+    MHIP(hipMemcpy(n->dX, x, sizeof(float) * rows * n->widths[0], hipMemcpyHostToDevice), "H2D");
+    MHIP(hipMemcpy(n->dY, t, sizeof(float) * rows * n->widths[n->L], hipMemcpyHostToDevice), "H2D");
+    n->ndata = rows;
+    return 0;
+}
+
+// Mini-batch Adam over the device data: epochs of shuffled batches.
+// losses[e] receives the mean training loss of epoch e.
+int ctl_mlp_train(ctl_mlp* n, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* losses) {
+    ctl::clear_error();
+    if (!n || epochs <= 0 || batch <= 0) return ctl::fail(CTL_E_ARGS, "bad arguments");
+    if (!n->ndata) return ctl::fail(CTL_E_STATE, "no training data");
+    MHIP(hipSetDevice(n->device), "hipSetDevice");
+    int rc = ensure_cap(n, batch);
+    if (rc) return rc;
+    std::mt19937_64 rng(seed);
+    std::vector<int32_t> perm(n->ndata);
+    for (long i = 0; i < n->ndata; ++i) perm[i] = (int32_t)i;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int ep = 0; ep < epochs; ++ep) {
+        std::shuffle(perm.begin(), perm.end(), rng);
+        double acc = 0.0;
+        long nb = 0;
+        for (long s0 = 0; s0 + batch <= n->ndata; s0 += batch) {
+            MHIP(hipMemcpyAsync(n->idx, perm.data() + s0, sizeof(int32_t) * batch, hipMemcpyHostToDevice, n->stream), "H2D");
+            const int w0 = n->widths[0], wL = n->widths[n->L];
+            hipLaunchKernelGGL(gather_rows, dim3((unsigned)(((long)batch * w0 + 255) / 256)), dim3(256), 0, n->stream,
+                               n->dX, n->idx, n->act[0], (float*)nullptr, batch, w0);
+            hipLaunchKernelGGL(gather_rows, dim3((unsigned)(((long)batch * wL + 255) / 256)), dim3(256), 0, n->stream,
+                               n->dY, n->idx, (uint16_t*)nullptr, n->tgt, batch, wL);
+            forward(n, batch);
+            const float l = loss_and_grad(n, batch, n->tgt, nullptr);
+            backward(n, batch);
+            n->step += 1;
+            const float c1 = 1.0f - std::pow(b1, (float)n->step), c2 = 1.0f - std::pow(b2, (float)n->step);
+            hipLaunchKernelGGL(adam, dim3((unsigned)((n->nparam + 255) / 256)), dim3(256), 0, n->stream, n->w, n->wb,
+                               n->g, n->m, n->v, n->nparam, lr, b1, b2, eps, c1, c2);
+            acc += l;
+            ++nb;
+        }
+        MHIP(hipStreamSynchronize(n->stream), "train");
+        if (losses) losses[ep] = nb ? (float)(acc / nb) : NAN;
+    }
+    return 0;
+}
+
+void ctl_mlp_destroy(ctl_mlp* n) {
+    if (!n) return;
+    (void)hipSetDevice(n->device);
+    for (void* q : n->allocs) (void)hipFree(q);
+    if (n->dX) (void)hipFree(n->dX);
+    if (n->dY) (void)hipFree(n->dY);
+    if (n->stream) (void)hipStreamDestroy(n->stream);
+    delete n;
+}
+
+}  // extern "C"

@@ -1,0 +1,179 @@
+"""Learned surrogate for parameter sweeps of a reaction network.
+
+A sweep asks, for many settings of uncertain rate constants, for the mean
+and standard deviation trajectories of observed species.  The plain answer
+runs `replicas` stochastic simulations per setting on the GPU and reduces
+them on the device (Sim.moments).  The surrogate answer is an ensemble of
+MLPs (cutauleaping_amd.MLP, BF16-MFMA kernels) mapping the log10 rate
+multipliers to those trajectories; settings on which the ensemble members
+disagree by more than a threshold fall back to the simulator, which is also
+the judge of the surrogate's error on held-out settings.
+
+Problem (defaults): a seeded mass-action network with N=50 species and
+M=100 reactions (networks.mass_action), 32 uncertain constants
+c_j = c_j^nom * 10^u_j with u_j ~ U(-1, 1), 10 observed species at 50
+equally spaced times.  Network features: u (32 values).  Targets: log1p of
+the per-time, per-species mean and (population) standard deviation over
+the replicas, 2 * 50 * 10 = 1000 values, standardised per output.
+
+The reference has no counterpart: it simulates every setting (its
+fitness/parameter-estimation drivers call the simulator once per
+candidate, kernel.cu:242-345).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import MLP, Model, Sim
+from . import networks
+
+WIDTHS = (32, 256, 256, 256, 1000)
+
+
+@dataclass
+class SweepProblem:
+    left: np.ndarray
+    right: np.ndarray
+    x0: np.ndarray
+    c_nom: np.ndarray
+    uncertain: np.ndarray      # reaction indices whose constants vary
+    observed: np.ndarray       # sampled species indices
+    ts: np.ndarray             # sample times (last = t_end)
+    seed: int = 0
+    info: dict = field(default_factory=dict)
+
+    @classmethod
+    def synthetic(cls, N=50, M=100, seed=5, n_uncertain=32, n_observed=10, K=50, t_end=5.0,
+                  decades=1.0) -> "SweepProblem":
+        """Draws the network (int32-safe at the nominal constants and with
+        every uncertain constant raised by 10^decades), the uncertain
+        reactions and the observed species from `seed`."""
+        for sub in range(100):
+            net = networks.mass_action(N, M, seed=seed * 1000 + sub, t_end=t_end)
+            rng = np.random.default_rng([seed, sub, 7])
+            unc = np.sort(rng.choice(M, size=n_uncertain, replace=False))
+            obs = np.sort(rng.choice(N, size=n_observed, replace=False))
+            hi = net["c"].copy()
+            hi[unc] *= 10.0 ** decades
+            if networks._mean_field_ok(net["left"], net["right"], net["x0"], hi, t_end):
+                break
+        else:
+            raise RuntimeError("no int32-safe sweep network found")
+        ts = np.arange(1, K + 1, dtype=np.float64) * (t_end / K)
+        return cls(net["left"], net["right"], net["x0"], net["c"], unc, obs, ts, seed,
+                   dict(N=N, M=M, sub_seed=seed * 1000 + sub, attempt=net["attempt"], decades=decades))
+
+    @property
+    def n_in(self) -> int:
+        return len(self.uncertain)
+
+    @property
+    def n_out(self) -> int:
+        return 2 * len(self.ts) * len(self.observed)
+
+    def sample_u(self, n: int, rng) -> np.ndarray:
+        d = self.info.get("decades", 1.0)
+        return rng.uniform(-d, d, size=(n, self.n_in)).astype(np.float32)
+
+    def rates(self, u) -> np.ndarray:
+        u = np.atleast_2d(np.asarray(u, np.float64))
+        c = np.tile(self.c_nom, (u.shape[0], 1))
+        c[:, self.uncertain] *= 10.0 ** u
+        return c
+
+    def simulate(self, u, replicas: int, *, offset: int = 0, seed: int = 0x5EEDC0DE, gpu: int = 0, **opts):
+        """Runs `replicas` simulations per row of u in one launch; returns
+        (mean, sd, stats, seconds) with mean/sd shaped [rows, K, observed]."""
+        u = np.atleast_2d(u)
+        rows = u.shape[0]
+        if replicas % 64:
+            raise ValueError("replicas must be a multiple of 64")
+        model = Model.from_arrays(self.left, self.right, self.x0, self.rates(u), self.ts, cs=self.observed)
+        t0 = time.perf_counter()
+        with Sim(model, 64, rows * replicas // 64, gpu=gpu, offset=offset, seed=seed, **opts) as s:
+            s.run()
+            mean, sd = s.moments(replicas)
+            st = s.stats()
+        dt = time.perf_counter() - t0
+        model.close()
+        return mean, sd, st, dt
+
+    def features(self, mean, sd) -> np.ndarray:
+        """[rows, n_out] float32 log1p targets: all means, then all sds."""
+        rows = mean.shape[0]
+        return np.concatenate([np.log1p(mean).reshape(rows, -1), np.log1p(sd).reshape(rows, -1)],
+                              axis=1).astype(np.float32)
+
+    def unfeatures(self, y):
+        rows = y.shape[0]
+        h = self.n_out // 2
+        K, SS = len(self.ts), len(self.observed)
+        m = np.expm1(np.maximum(y[:, :h], 0.0)).reshape(rows, K, SS)
+        s = np.expm1(np.maximum(y[:, h:], 0.0)).reshape(rows, K, SS)
+        return m, s
+
+
+def relative_error(hat, ref) -> np.ndarray:
+    """Per-row relative L2 error ||hat - ref|| / ||ref|| over all times and species."""
+    hat = hat.reshape(hat.shape[0], -1)
+    ref = ref.reshape(ref.shape[0], -1)
+    return np.linalg.norm(hat - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-12)
+
+
+class Surrogate:
+    """An ensemble of BF16-MFMA MLPs over standardised targets."""
+
+    def __init__(self, widths=WIDTHS, members: int = 5, gpu: int = 0, seed: int = 1):
+        self.widths = tuple(int(w) for w in widths)
+        self.nets = [MLP(self.widths, gpu=gpu, seed=seed * 101 + k) for k in range(members)]
+        self.mu = None
+        self.sigma = None
+
+    def fit(self, u, y, *, epochs=200, batch=256, lr=1e-3, seed=0):
+        """Trains every member on (u, y) (y from SweepProblem.features);
+        returns the per-epoch losses, [members, epochs]."""
+        y = np.asarray(y, np.float32)
+        self.mu = y.mean(axis=0)
+        self.sigma = np.maximum(y.std(axis=0), 1e-3).astype(np.float32)
+        z = ((y - self.mu) / self.sigma).astype(np.float32)
+        losses = []
+        for k, net in enumerate(self.nets):
+            net.set_data(u, z)
+            losses.append(net.train(epochs, batch, lr, seed=seed * 31 + k))
+        return np.stack(losses)
+
+    def predict(self, u):
+        """(y_mean [rows, n_out], spread [rows]): the ensemble mean in target
+        units and the RMS over outputs of the members' standard deviation,
+        also in target (log1p) units, i.e. about a relative spread."""
+        zs = np.stack([net.forward(u) for net in self.nets])
+        ys = zs * self.sigma + self.mu
+        spread = np.sqrt(np.mean(ys.var(axis=0), axis=1))
+        return ys.mean(axis=0), spread
+
+    def close(self):
+        for n in self.nets:
+            n.close()
+
+
+def screened_sweep(problem: SweepProblem, surrogate: Surrogate, u, *, replicas: int, threshold: float,
+                   offset: int = 0, seed: int = 0x5EEDC0DE, **opts):
+    """Answers a sweep with the surrogate, simulating the rows whose ensemble
+    spread exceeds `threshold`.  Returns dict(mean, sd, fallback (bool rows),
+    seconds, surrogate_seconds, simulator_seconds)."""
+    t0 = time.perf_counter()
+    y, spread = surrogate.predict(u)
+    mean, sd = problem.unfeatures(y)
+    t1 = time.perf_counter()
+    fb = spread > threshold
+    sim_s = 0.0
+    if fb.any():
+        m2, s2, _, sim_s = problem.simulate(u[fb], replicas, offset=offset, seed=seed, **opts)
+        mean[fb] = m2
+        sd[fb] = s2
+    t2 = time.perf_counter()
+    return dict(mean=mean, sd=sd, fallback=fb, spread=spread, seconds=t2 - t0, surrogate_seconds=t1 - t0,
+                simulator_seconds=sim_s)

@@ -219,6 +219,36 @@ int ctl_batch_stats(const ctl_batch* b, ctl_stats* out);
 int ctl_batch_stream_base(const ctl_batch* b, int32_t model, int64_t* base);
 void ctl_batch_destroy(ctl_batch* b);
 
+/* ---- surrogate MLP ------------------------------------------------------- */
+/* A multilayer perceptron widths[0] -> ... -> widths[nlayers] with ReLU on the
+ * hidden layers and a linear output, trained and evaluated by hand-written
+ * BF16-MFMA kernels with FP32 accumulation.  No counterpart in the reference:
+ * it is the learned surrogate that screens parameter sweeps (log rate
+ * constants -> per-species mean and standard deviation trajectories); the
+ * simulator above generates its data and judges its error.
+ * Parameters are one flat FP32 vector: W_1 [w0][w1] row-major, b_1 [w1],
+ * W_2, b_2, ...  Weights start He-uniform from seed, biases at zero. */
+typedef struct ctl_mlp ctl_mlp;
+int ctl_mlp_create(const int32_t* widths, int32_t nlayers, int32_t gpu, uint64_t seed, ctl_mlp** out);
+int ctl_mlp_num_params(const ctl_mlp* n, int64_t* count);
+/* set != 0: upload count FP32 parameters; set == 0: download them. */
+int ctl_mlp_params(ctl_mlp* n, float* host, size_t count, int32_t set);
+/* y [rows][wL] = f(x [rows][w0]); host buffers. */
+int ctl_mlp_forward(ctl_mlp* n, const float* x, int64_t rows, float* y);
+/* Same on device buffers, enqueued on the network's stream and waited for;
+ * ms (optional) receives the device time of the forward kernels. */
+int ctl_mlp_forward_device(ctl_mlp* n, const float* x_dev, int64_t rows, float* y_dev, float* ms);
+/* Loss (mean squared error over rows*wL) and its FP32 gradient with respect
+ * to every parameter (grad may be NULL) for one batch of rows <= 65536. */
+int ctl_mlp_gradient(ctl_mlp* n, const float* x, const float* t, int32_t rows, float* grad, size_t count,
+                     float* loss);
+/* Training set kept in device memory: x [rows][w0], t [rows][wL]. */
+int ctl_mlp_set_data(ctl_mlp* n, const float* x, const float* t, int64_t rows);
+/* Mini-batch Adam (beta1 0.9, beta2 0.999, eps 1e-8) over shuffled batches;
+ * losses[e] (optional) receives the mean batch loss of epoch e. */
+int ctl_mlp_train(ctl_mlp* n, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* losses);
+void ctl_mlp_destroy(ctl_mlp* n);
+
 /* ---- misc -------------------------------------------------------------- */
 const char* ctl_last_error(void);
 const char* ctl_version(void);
