@@ -24,26 +24,35 @@ import numpy as np
 
 
 def _mean_field_ok(left, right, x0, c, t_end, limit=1e8):
-    """Integrates dx/dt = (right-left)^T a(x) with a_j = c_j prod C(x, v)."""
+    """Integrates dx/dt = (right-left)^T a(x) with a_j = c_j prod C(x, v)
+    (orders 0-2: one reactant, a pair of distinct reactants, or a dimer)."""
     from scipy.integrate import solve_ivp
 
     V = (right - left).astype(np.float64)
     R, S = left.shape
-
-    def rates(x):
-        a = c.copy()
-        for j in range(R):
-            for s in np.nonzero(left[j])[0]:
-                v = left[j, s]
-                xs = max(x[s], 0.0)
-                if v == 1:
-                    a[j] *= xs
-                else:
-                    a[j] *= xs * max(xs - 1, 0.0) / 2.0
-        return a
+    if left.sum(axis=1).max(initial=0) > 2:
+        raise ValueError("mean-field check supports reaction orders 0-2")
+    # reactant slots: p[j], q[j] species (or S for "none", whose count is 1);
+    # dimers (coefficient 2) use x (x - 1) / 2
+    p = np.full(R, S)
+    q = np.full(R, S)
+    dimer = np.zeros(R, bool)
+    for j in range(R):
+        nz = np.nonzero(left[j])[0]
+        if len(nz) == 1 and left[j, nz[0]] == 2:
+            p[j] = q[j] = nz[0]
+            dimer[j] = True
+        elif len(nz) >= 1:
+            p[j] = nz[0]
+            if len(nz) == 2:
+                q[j] = nz[1]
+    VT = V.T.copy()
 
     def f(t, x):
-        return V.T @ rates(x)
+        xe = np.append(np.maximum(x, 0.0), 1.0)
+        xp, xq = xe[p], xe[q]
+        xq = np.where(dimer, np.maximum(xq - 1.0, 0.0) / 2.0, xq)
+        return VT @ (c * xp * xq)
 
     def blowup(t, x):
         return limit - np.max(x)

@@ -103,17 +103,31 @@ def test_training_fits_smooth_map(ctl, gpu):
 
 
 def test_forward_device_matches_host(ctl, gpu):
-    import torch
+    """Device buffers come from the HIP runtime libctl itself links (a second
+    runtime in the process, such as the one bundled with a torch wheel,
+    cannot share pointers with it)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    H2D, D2H = 1, 2
     widths = [32, 256, 256, 256, 1000]
     net = ctl.MLP(widths, seed=9)
     x = np.random.default_rng(2).uniform(-1, 1, (70000, 32)).astype(np.float32)   # > one 65536 chunk
-    xd = torch.from_numpy(x).to("cuda:0")
-    yd = torch.empty((x.shape[0], widths[-1]), dtype=torch.float32, device="cuda:0")
-    torch.cuda.synchronize()
-    ms = net.forward_device(xd.data_ptr(), x.shape[0], yd.data_ptr())
-    assert ms > 0
-    yh = net.forward(x)
-    assert np.array_equal(yd.cpu().numpy(), yh)
+    y = np.zeros((x.shape[0], widths[-1]), np.float32)
+    xd, yd = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(xd), x.nbytes) == 0
+    assert hip.hipMalloc(C.byref(yd), y.nbytes) == 0
+    try:
+        assert hip.hipMemcpy(xd, x.ctypes.data, x.nbytes, H2D) == 0
+        ms = net.forward_device(xd.value, x.shape[0], yd.value)
+        assert ms > 0
+        assert hip.hipMemcpy(y.ctypes.data, yd, y.nbytes, D2H) == 0
+    finally:
+        hip.hipFree(xd)
+        hip.hipFree(yd)
+    assert np.array_equal(y, net.forward(x))
     net.close()
 
 
