@@ -140,8 +140,7 @@ def main():
                              "peak_GBps": HBM_PEAK / 1e9,
                              "frac": hbm_bytes / (ker_ms / 1000.0 / args.steps) / HBM_PEAK}},
         "counters": {k: last[k] for k in ("firings", "leaps", "rejections", "switches", "ssa_steps",
-                                          "poisson", "gaps", "absorbed"), "lds_state": last["lds_state"],
-                     "slots": last["slots"], "block_size": last["block_size"]},
+                                          "poisson", "gaps", "absorbed", "lds_state", "slots", "block_size")},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(net, ts, args.cpu_sims)
