@@ -43,7 +43,7 @@ class orc_job(C.Structure):
                 ("sim_base", C.c_uint32),
                 ("nc", C.c_int), ("eps", C.c_float), ("ssa_steps", C.c_int),
                 ("ssa_factor", C.c_float), ("force_ssa", C.c_int),
-                ("precision", C.c_int), ("threads", C.c_int)]
+                ("precision", C.c_int), ("threads", C.c_int), ("group", C.c_int)]
 
 
 class orc_stat(C.Structure):
@@ -54,7 +54,8 @@ class orc_stat(C.Structure):
 
 class orc_model(C.Structure):
     _fields_ = [("S", C.c_int), ("R", C.c_int), ("nL", C.c_int), ("nV", C.c_int), ("nF", C.c_int)] + [
-        (n, C.POINTER(C.c_int)) for n in ("Lptr", "Ls", "Lv", "Vptr", "Vr", "Vv", "Fptr", "Fs", "Fv")
+        (n, C.POINTER(C.c_int)) for n in ("Lptr", "Ls", "Lv", "Vptr", "Vr", "Vv", "Fptr", "Fs", "Fv",
+                                          "FSptr", "FSr", "FSv")
     ] + [("hor", C.POINTER(C.c_ubyte)), ("hort", C.POINTER(C.c_ubyte))]
 
 
@@ -132,9 +133,10 @@ def compile_model(left, right, locked=None) -> dict:
 
 def run(left, right, x0, c, ts, *, locked=None, cs=None, n=None, seed=1, sim_base=0,
         nc=10, eps=0.03, ssa_steps=100, ssa_factor=10.0, force_ssa=False,
-        precision=REPLAY, threads=0, final=False):
+        precision=REPLAY, threads=0, final=False, group=0):
     """Runs n simulations; returns (samples[n, K, SS] uint32, stats structured
-    array) and, with final=True, also the final states [n, S] int32."""
+    array) and, with final=True, also the final states [n, S] int32.
+    group=G > 0 runs the group algorithm with G lanes per simulation."""
     x0 = np.ascontiguousarray(np.atleast_2d(x0), dtype=np.uint32)
     c = np.ascontiguousarray(np.atleast_2d(c), dtype=np.float64)
     ts = np.ascontiguousarray(ts, dtype=np.float64)
@@ -148,7 +150,7 @@ def run(left, right, x0, c, ts, *, locked=None, cs=None, n=None, seed=1, sim_bas
     job = orc_job(n, _ptr(x0, C.c_uint32), x0.shape[0], _ptr(c, C.c_double), c.shape[0],
                   _ptr(ts, C.c_double), len(ts), _ptr(cs, C.c_int32), len(cs),
                   seed & 0xffffffff, (seed >> 32) & 0xffffffff, sim_base,
-                  nc, eps, ssa_steps, ssa_factor, int(bool(force_ssa)), precision, threads)
+                  nc, eps, ssa_steps, ssa_factor, int(bool(force_ssa)), precision, threads, group)
     out = np.zeros((n, len(ts), len(cs)), dtype=np.uint32)
     st = np.zeros(n, dtype=STAT_DTYPE)
     assert st.dtype.itemsize == C.sizeof(orc_stat)

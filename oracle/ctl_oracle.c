@@ -54,13 +54,14 @@ typedef struct {
     uint32_t sim, ev, blk;
     uint32_t buf[4];
     int pos;
+    uint32_t tag;        /* counter word 3: 0, or 1 + lane for the group mode's lane streams */
 } orc_rng;
 
 static inline void rng_begin(orc_rng* g, uint32_t ev) { g->ev = ev; g->blk = 0; g->pos = 4; }
 
 static inline uint32_t rng_next(orc_rng* g) {
     if (g->pos == 4) {
-        uint32_t ctr[4] = {g->blk, g->ev, g->sim, 0u};
+        uint32_t ctr[4] = {g->blk, g->ev, g->sim, g->tag};
         orc_philox(ctr, g->key, g->buf);
         g->blk++;
         g->pos = 0;
@@ -262,14 +263,14 @@ static int32_t poisson(orc_rng* g, float lam) {
 
 void orc_poisson_stream(uint32_t seed_lo, uint32_t seed_hi, uint32_t sim, uint32_t ev,
                         float lam, int count, int32_t* out) {
-    orc_rng g = {{seed_lo, seed_hi}, sim, 0, 0, {0, 0, 0, 0}, 4};
+    orc_rng g = {{seed_lo, seed_hi}, sim, 0, 0, {0, 0, 0, 0}, 4, 0u};
     rng_begin(&g, ev);
     for (int i = 0; i < count; ++i) out[i] = poisson(&g, lam);
 }
 
 void orc_uniform_stream(uint32_t seed_lo, uint32_t seed_hi, uint32_t sim, uint32_t ev,
                         int open_closed, int count, float* out) {
-    orc_rng g = {{seed_lo, seed_hi}, sim, 0, 0, {0, 0, 0, 0}, 4};
+    orc_rng g = {{seed_lo, seed_hi}, sim, 0, 0, {0, 0, 0, 0}, 4, 0u};
     rng_begin(&g, ev);
     for (int i = 0; i < count; ++i) out[i] = open_closed ? u01_oc(&g) : u01_co(&g);
 }
@@ -281,6 +282,7 @@ void orc_model_free(orc_model* m) {
     free(m->Lptr); free(m->Ls); free(m->Lv);
     free(m->Vptr); free(m->Vr); free(m->Vv);
     free(m->Fptr); free(m->Fs); free(m->Fv);
+    free(m->FSptr); free(m->FSr); free(m->FSv);
     free(m->hor); free(m->hort);
     memset(m, 0, sizeof(*m));
 }
@@ -327,6 +329,19 @@ int orc_compile(const orc_net* net, orc_model* m) {
             if (var[r * S + s]) { m->Vr[nv] = r; m->Vv[nv] = var[r * S + s]; nv++; }
     }
     m->Vptr[S] = nv;
+    /* fear entries species-major, ascending reaction (group algorithm update) */
+    m->FSptr = (int*)malloc(sizeof(int) * (S + 1));
+    m->FSr = (int*)malloc(sizeof(int) * (m->nF + 1));
+    m->FSv = (int*)malloc(sizeof(int) * (m->nF + 1));
+    {
+        int nfs = 0;
+        for (int sp = 0; sp < S; ++sp) {
+            m->FSptr[sp] = nfs;
+            for (int r = 0; r < R; ++r)
+                if (fear[r * S + sp]) { m->FSr[nfs] = r; m->FSv[nfs] = fear[r * S + sp]; nfs++; }
+        }
+        m->FSptr[S] = nfs;
+    }
     /* HOR / HORtype, T.cu:402-432: reaction order = sum of left coefficients
      * (unsigned char); the type is overwritten when a higher order is found and
      * then raised to any larger left coefficient (no braces in the reference). */
@@ -482,31 +497,36 @@ static void swap_state(simctx* s) { int32_t* tmp = s->X; s->X = s->X1; s->X1 = t
 
 /* CalculatePropensity<DEV_FLATTEN_MATRIX_CONST>, T.cu:1443-1470, in FP32;
  * criticality of T.cu:1725-1735 fused in: 0 < floor(X/v) < nc. */
-static float prop_f32(simctx* s, float* a0c) {
+static float prop_one_f32(simctx* s, int j, unsigned char* crit_out) {
     const orc_model* m = s->m;
     const uint32_t nc = (uint32_t)s->J->nc;
+    float a = s->cf[j];
+    int crit = 0;
+    for (int h = m->Lptr[j]; h < m->Lptr[j + 1]; ++h) {
+        int32_t x = s->X[m->Ls[h]];
+        int v = m->Lv[h];
+        if (v == 1) {
+            a = a * (float)x;
+        } else if (v == 2) {
+            a = a * ((x < 2) ? 0.0f : (0.5f * (float)x) * (float)(x - 1));
+        } else if (x < v) {
+            a = a * 0.0f;
+        } else {
+            for (int e = 1; e <= v; ++e) a = a * ((float)(x - e + 1) / (float)e);
+        }
+        uint32_t q = (uint32_t)x / (uint32_t)v;
+        if (q > 0u && q < nc) crit = 1;
+    }
+    *crit_out = (unsigned char)crit;
+    return a;
+}
+
+static float prop_f32(simctx* s, float* a0c) {
+    const orc_model* m = s->m;
     float a0 = 0.0f, ac = 0.0f;
     for (int j = 0; j < m->R; ++j) {
-        float a = s->cf[j];
-        int crit = 0;
-        for (int h = m->Lptr[j]; h < m->Lptr[j + 1]; ++h) {
-            int32_t x = s->X[m->Ls[h]];
-            int v = m->Lv[h];
-            if (v == 1) {
-                a = a * (float)x;
-            } else if (v == 2) {
-                a = a * ((x < 2) ? 0.0f : (0.5f * (float)x) * (float)(x - 1));
-            } else if (x < v) {
-                a = a * 0.0f;
-            } else {
-                for (int e = 1; e <= v; ++e) a = a * ((float)(x - e + 1) / (float)e);
-            }
-            uint32_t q = (uint32_t)x / (uint32_t)v;
-            if (q > 0u && q < nc) crit = 1;
-        }
-        s->Af[j] = a;
-        s->C[j] = (unsigned char)crit;
-        a0 = a0 + a;
+        s->Af[j] = prop_one_f32(s, j, &s->C[j]);
+        a0 = a0 + s->Af[j];
     }
     for (int j = 0; j < m->R; ++j)            /* T.cu:1945-1952 */
         if (s->C[j]) ac = ac + s->Af[j];
@@ -765,6 +785,206 @@ static void run_one(simctx* s) {
     s->st.events = s->evc;
 }
 
+/* ====================== GROUP algorithm (G lanes) ===================== */
+/* The gfx950 group kernel's algorithm, lane by lane (see the header).     */
+
+typedef struct {
+    simctx* s;
+    int G;
+    orc_rng* lane;       /* G lane streams, tag 1 + l */
+    int32_t* K;          /* R drawn counts */
+    float *P, *Q, *T;    /* G lane partials and scan scratch */
+} grpctx;
+
+/* Hillis-Steele inclusive scan over G lanes: v[l] += v[l - d], d = 1, 2, 4... */
+static void g_scan(float* v, float* tmp, int G) {
+    for (int d = 1; d < G; d <<= 1) {
+        memcpy(tmp, v, sizeof(float) * (size_t)G);
+        for (int l = d; l < G; ++l) v[l] = tmp[l - d] + tmp[l];
+    }
+}
+
+/* Lane partials of the propensities (all, or the critical-pick weights). */
+static void g_partials(grpctx* q, float* P, int which) {
+    const simctx* s = q->s;
+    const int G = q->G, R = s->m->R;
+    for (int l = 0; l < G; ++l) {
+        float p = 0.0f;
+        for (int j = l; j < R; j += G) {
+            if (which == 1 && !s->C[j]) continue;                 /* critical */
+            if (which == 2 && !(s->C[j] || j == 0)) continue;     /* pick walk: reaction 0 first */
+            p = p + s->Af[j];
+        }
+        P[l] = p;
+    }
+}
+
+/* First lane whose inclusive sum reaches w, then the walk inside it. */
+static int g_select(grpctx* q, const float* P, float w, int which) {
+    const simctx* s = q->s;
+    const int G = q->G, R = s->m->R;
+    int L = -1;
+    for (int l = 0; l < G; ++l)
+        if (P[l] >= w) { L = l; break; }
+    if (L < 0) return -1;
+    float acc = L > 0 ? P[L - 1] : 0.0f;
+    int last = -1;
+    for (int j = L; j < R; j += G) {
+        if (which == 2 && !(s->C[j] || j == 0)) continue;
+        const float a = s->Af[j];
+        acc = acc + a;
+        if (a > 0.0f) last = j;
+        if (w <= acc) return j;
+    }
+    return last;
+}
+
+static int g_leap(grpctx* q) {
+    simctx* s = q->s;
+    const orc_model* m = s->m;
+    const orc_job* J = s->J;
+    const int G = q->G, R = m->R, S = m->S;
+    if (s->t >= s->tmax) { s->mode = MODE_DONE; return LEAP_DONE; }
+    s->mode = MODE_LEAP;
+    for (int j = 0; j < R; ++j) s->Af[j] = prop_one_f32(s, j, &s->C[j]);
+    g_partials(q, q->P, 0);
+    g_scan(q->P, q->T, G);
+    g_partials(q, q->Q, 1);
+    g_scan(q->Q, q->T, G);
+    const float a0 = q->P[G - 1], a0c = q->Q[G - 1];
+    if (a0 == 0.0f) { absorb(s); return LEAP_ABSORB; }
+    /* tau': the REPLAY per-species values; their minimum is order-free */
+    float tau1 = FLT_MAX;
+    for (int sp = 0; sp < S; ++sp) {
+        float mu = 0.0f, sg = 0.0f;
+        for (int h = m->Vptr[sp]; h < m->Vptr[sp + 1]; ++h) {
+            int j = m->Vr[h];
+            if (s->C[j]) continue;
+            int v = m->Vv[h];
+            mu = mu + (float)v * s->Af[j];
+            sg = sg + (float)(v * v) * s->Af[j];
+        }
+        if (mu == 0.0f || sg == 0.0f) continue;
+        float gg = g_f32(m->hor[sp], m->hort[sp], s->X[sp]);
+        float tmp2 = (J->eps * (float)s->X[sp]) / gg;
+        float macs = tmp2 > 1.0f ? tmp2 : 1.0f;
+        float t3 = macs / fabsf(mu);
+        float t4 = (macs * macs) / sg;
+        float tmp = t3 < t4 ? t3 : t4;
+        if (tmp < tau1) tau1 = tmp;
+    }
+    if (tau1 < J->ssa_factor / a0) { s->mode = MODE_SSA; s->st.switches++; return LEAP_SWITCH; }
+    uint32_t e = s->evc++;
+    rng_begin(&s->g, e);
+    float tau2 = FLT_MAX;
+    if (a0c > 0.0f) tau2 = exp_rv(&s->g) / a0c;
+    float tau = tau1 < tau2 ? tau1 : tau2;
+    int pick = -1;
+    if (tau1 > tau2) {
+        const float w = u01_oc(&s->g) * a0c;
+        g_partials(q, q->Q, 2);
+        g_scan(q->Q, q->T, G);
+        pick = g_select(q, q->Q, w, 2);
+    }
+    uint64_t fired;
+    for (int attempt = 0;; ++attempt) {
+        if (attempt > 0) { e = s->evc++; s->st.rejections++; }
+        fired = 0;
+        for (int l = 0; l < G; ++l) {
+            orc_rng* g = &q->lane[l];
+            rng_begin(g, e);
+            double cum = 0.0, pos = -1.0;
+            for (int j = l; j < R; j += G) {
+                int32_t k;
+                if (s->C[j]) {
+                    k = (attempt == 0 && j == pick) ? 1 : 0;
+                } else {
+                    const float lam = tau * s->Af[j];
+                    if (lam < 1.0f) {
+                        k = 0;
+                        if (lam > 0.0f) {
+                            if (pos < 0.0) { pos = (double)exp_rv(g); s->st.gaps++; }
+                            cum = cum + (double)lam;
+                            while (pos < cum) { ++k; pos = pos + (double)exp_rv(g); s->st.gaps++; }
+                        }
+                    } else {
+                        k = poisson(g, lam);
+                        s->st.poisson++;
+                    }
+                }
+                q->K[j] = k;
+                fired += (uint64_t)k;
+            }
+        }
+        int ok = 1;
+        for (int sp = 0; sp < S; ++sp) {
+            int64_t x = s->X[sp];
+            for (int h = m->FSptr[sp]; h < m->FSptr[sp + 1]; ++h) {
+                x += (int64_t)m->FSv[h] * (int64_t)q->K[m->FSr[h]];
+                if (x < 0 || x > 2147483647LL) { ok = 0; break; }
+            }
+            s->X1[sp] = (int32_t)x;
+        }
+        if (ok) break;
+        tau = tau * 0.5f;
+    }
+    leap_samples(s, (double)tau);
+    swap_state(s);
+    s->t = s->t + (double)tau;
+    s->st.firings += fired;
+    s->st.leaps++;
+    return LEAP_OK;
+}
+
+static void g_ssa(grpctx* q) {
+    simctx* s = q->s;
+    const orc_model* m = s->m;
+    const orc_job* J = s->J;
+    const int G = q->G, R = m->R;
+    rng_begin(&s->g, s->evc++);
+    double tacc = 0.0;
+    for (int step = 0; step < J->ssa_steps; ++step) {
+        for (int j = 0; j < R; ++j) s->Af[j] = prop_one_f32(s, j, &s->C[j]);
+        g_partials(q, q->P, 0);
+        g_scan(q->P, q->T, G);
+        const float a0 = q->P[G - 1];
+        if (a0 == 0.0f) { absorb(s); return; }
+        const float r1 = u01_oc(&s->g) * a0;
+        int j = g_select(q, q->P, r1, 0);
+        const float dt = exp_rv(&s->g) / a0;
+        tacc = tacc + (double)dt;
+        const double tn = s->t + tacc;
+        ssa_samples(s, tn);
+        for (int h = m->Fptr[j]; h < m->Fptr[j + 1]; ++h)
+            s->X[m->Fs[h]] = (int32_t)((uint32_t)s->X[m->Fs[h]] + (uint32_t)m->Fv[h]);
+        s->st.ssa_steps++;
+        if (tn <= s->tmax) s->st.firings++;
+    }
+    s->t = s->t + tacc;
+}
+
+static void run_group(grpctx* q) {
+    simctx* s = q->s;
+    const orc_job* J = s->J;
+    s->mode = J->force_ssa ? MODE_SSA : MODE_LEAP;
+    for (;;) {
+        if (J->force_ssa) {
+            if (s->t >= s->tmax) break;
+            g_ssa(q);
+            if (s->mode == MODE_DONE) break;
+            continue;
+        }
+        int r = g_leap(q);
+        if (r == LEAP_DONE || r == LEAP_ABSORB) break;
+        if (r == LEAP_SWITCH) {
+            g_ssa(q);
+            if (s->mode == MODE_DONE) break;
+        }
+    }
+    s->st.t_end = s->t;
+    s->st.events = s->evc;
+}
+
 /* Input row of simulation i: one shared row, one row per simulation
  * (rows >= n, the reference's MX_0 / c_matrix), or n / rows consecutive
  * simulations per row (replica groups). */
@@ -777,6 +997,8 @@ static int row_of(int rows, int n, int i) {
 int orc_run(const orc_net* net, const orc_job* J, uint32_t* storage, orc_stat* stats, int32_t* xfinal) {
     orc_model m;
     if (J->K <= 0 || J->SS < 0 || J->n < 0) return -1;
+    if (J->group < 0 || J->group > 64 || (J->group & (J->group - 1))) return -1;
+    if (J->group > 0 && J->precision != ORC_REPLAY) return -1;
     if ((J->x0_rows > 1 && J->x0_rows < J->n && J->n % J->x0_rows) ||
         (J->c_rows > 1 && J->c_rows < J->n && J->n % J->c_rows))
         return -1;
@@ -796,6 +1018,10 @@ int orc_run(const orc_net* net, const orc_job* J, uint32_t* storage, orc_stat* s
         double* Ad = (double*)malloc(sizeof(double) * R);
         float* cf = (float*)malloc(sizeof(float) * R);
         unsigned char* C = (unsigned char*)malloc(R);
+        const int G = J->group > 0 ? J->group : 1;
+        orc_rng* lanes = (orc_rng*)calloc((size_t)G, sizeof(orc_rng));
+        int32_t* Kc = (int32_t*)malloc(sizeof(int32_t) * R);
+        float* gP = (float*)malloc(sizeof(float) * 3 * (size_t)G);
 #pragma omp for schedule(dynamic, 16)
         for (int i = 0; i < J->n; ++i) {
             simctx s;
@@ -813,13 +1039,23 @@ int orc_run(const orc_net* net, const orc_job* J, uint32_t* storage, orc_stat* s
             s.g.key[0] = J->seed_lo; s.g.key[1] = J->seed_hi;
             s.g.sim = J->sim_base + (uint32_t)i;
             s.g.pos = 4;
-            run_one(&s);
+            if (J->group > 0) {
+                grpctx q = {&s, G, lanes, Kc, gP, gP + G, gP + 2 * G};
+                for (int l = 0; l < G; ++l) {
+                    lanes[l] = s.g;
+                    lanes[l].tag = 1u + (uint32_t)l;
+                }
+                run_group(&q);
+            } else {
+                run_one(&s);
+            }
             if (stats) stats[i] = s.st;
             if (xfinal) memcpy(xfinal + (size_t)i * S, s.X, sizeof(int32_t) * S);
             /* run_one may leave X/X1 swapped; keep the buffers this thread owns */
             if (s.X != X) { int32_t* tmp = X; X = X1; X1 = tmp; }
         }
         free(X); free(X1); free(Af); free(Ad); free(cf); free(C);
+        free(lanes); free(Kc); free(gP);
     }
     orc_model_free(&m);
     return 0;

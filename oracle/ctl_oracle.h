@@ -16,6 +16,16 @@
  * Exact SSA is the force_ssa mode of either precision (100-step direct
  * method bursts, 2phase_n-tau-leaping.cu:2280-2454).
  *
+ * Group algorithm (job.group = G): the gfx950 kernel for larger networks,
+ * where G lanes cooperate on one simulation.  Reaction j and species s
+ * belong to lane j mod G and s mod G.  Same per-reaction propensities and
+ * per-species tau candidates as REPLAY; a0 and the selection sums are the
+ * lane partial sums (ascending reactions) combined by a Hillis-Steele
+ * inclusive scan over the G lanes; each lane draws its own reactions'
+ * counts from its own Philox stream (counter word 3 = 1 + lane); the
+ * update and its negativity test run per species over the species-major
+ * fear entries (the same rejections as the reaction-major order).
+ *
  * Random numbers: Philox4x32-10 (Salmon et al., SC'11) keyed by the seed,
  * counter = {block, event, global simulation index, 0}; cuRAND's XORWOW
  * and curand_poisson (the reference's closed RNG) are replaced by the
@@ -54,6 +64,9 @@ typedef struct {
     int force_ssa;
     int precision;                  /* ORC_REPLAY | ORC_REFSEM */
     int threads;                    /* OpenMP threads, 0 = runtime default */
+    int group;                      /* 0: one lane per simulation (REPLAY / REFSEM);
+                                       G in {2,4,...,64}: the group algorithm with G lanes
+                                       per simulation (FP32, REPLAY precision) */
 } orc_job;
 
 typedef struct {
@@ -76,6 +89,7 @@ typedef struct {
     int *Lptr, *Ls, *Lv;        /* left flat, reaction-major (T.cu:219-232) */
     int *Vptr, *Vr, *Vv;        /* rav: var flat, species-major (T.cu:308-320) */
     int *Fptr, *Fs, *Fv;        /* fear flat, reaction-major (T.cu:323-335) */
+    int *FSptr, *FSr, *FSv;     /* the same entries species-major, ascending reaction */
     unsigned char *hor, *hort;  /* HOR / HORtype (T.cu:402-432) */
 } orc_model;
 
