@@ -77,7 +77,8 @@ class ctl_opts(C.Structure):
                 ("offset", C.c_int64), ("force_ssa", C.c_int32), ("nc", C.c_int32),
                 ("eps", C.c_float), ("ssa_steps", C.c_int32), ("ssa_factor", C.c_float),
                 ("seed", C.c_uint64), ("verbose", C.c_int32), ("block_size", C.c_int32),
-                ("slots", C.c_int32), ("iters_per_launch", C.c_int32), ("ssa_chunk", C.c_int32)]
+                ("slots", C.c_int32), ("iters_per_launch", C.c_int32), ("ssa_chunk", C.c_int32),
+                ("lanes_per_sim", C.c_int32)]
 
 
 class ctl_stats(C.Structure):
@@ -85,7 +86,8 @@ class ctl_stats(C.Structure):
                 ("switches", C.c_uint64), ("ssa_steps", C.c_uint64), ("poisson", C.c_uint64),
                 ("gaps", C.c_uint64), ("simulations", C.c_uint64), ("absorbed", C.c_uint64),
                 ("kernel_ms", C.c_double), ("wall_ms", C.c_double), ("launches", C.c_int32),
-                ("slots", C.c_int32), ("block_size", C.c_int32), ("lds_state", C.c_int32)]
+                ("slots", C.c_int32), ("block_size", C.c_int32), ("lds_state", C.c_int32),
+                ("lanes_per_sim", C.c_int32)]
 
 
 class ctl_model_info(C.Structure):
@@ -251,13 +253,14 @@ class Sim:
 
     def __init__(self, model: Model, threads: int, blocks: int, *, gpu=0, offset=0, force_ssa=False,
                  seed=0x5EEDC0DE, nc=10, eps=0.03, ssa_steps=100, ssa_factor=10.0, block_size=0,
-                 slots=0, iters_per_launch=0, ssa_chunk=0, verbose=False):
+                 slots=0, iters_per_launch=0, ssa_chunk=0, lanes_per_sim=0, verbose=False):
         o = default_opts()
         o.threads, o.blocks, o.gpu, o.offset = threads, blocks, gpu, offset
         o.force_ssa = int(bool(force_ssa))
         o.seed = seed
         o.nc, o.eps, o.ssa_steps, o.ssa_factor = nc, eps, ssa_steps, ssa_factor
         o.block_size, o.slots, o.iters_per_launch, o.ssa_chunk = block_size, slots, iters_per_launch, ssa_chunk
+        o.lanes_per_sim = lanes_per_sim
         o.verbose = int(bool(verbose))
         self.model = model
         self.n = threads * blocks

@@ -23,6 +23,7 @@ struct Rng {
     uint32_t blk;        // next block of the event
     uint32_t b0, b1, b2, b3;
     uint32_t pos;        // 4 = buffer empty
+    uint32_t tag = 0u;   // counter word 3: 0, or 1 + lane for a group's lane streams
 };
 
 __device__ __forceinline__ void philox_block(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
@@ -56,7 +57,7 @@ __device__ __forceinline__ void rng_event(Rng& g, uint32_t ev) {
 // always the next word), which keeps the whole state in VGPRs.
 __device__ __forceinline__ uint32_t rng_u32(Rng& g) {
     if (g.pos >= 4) {
-        philox_block(g.blk, g.ev, g.sim, 0u, g.k0, g.k1, g.b0, g.b1, g.b2, g.b3);
+        philox_block(g.blk, g.ev, g.sim, g.tag, g.k0, g.k1, g.b0, g.b1, g.b2, g.b3);
         g.blk += 1;
         g.pos = 0;
     }

@@ -1,0 +1,512 @@
+// ctl_group.h -- the group kernel: G lanes of a wavefront (G = 2..64, a
+// power of two) cooperate on one simulation, for networks whose per-lane
+// working set does not fit on chip.  Included by ctl_sim.hip.
+//
+// Work split: reaction j and species s belong to lane j mod G and s mod G.
+// A group's simulation state lives in LDS for the whole launch:
+//     X   int32 [2][S]   state and trial state (parity bit swaps them)
+//     A   uint32[R]      propensity (FP32 bits) | critical flag << 31
+//     Kc  int32 [R]      counts drawn for the current leap attempt
+//     Cs  float [R]      rate constants of the simulation
+// so every table-driven access is an LDS access instead of a dependent HBM
+// load.  Group-wide values (a0, tau, the chosen reaction) are computed with
+// cross-lane shuffles and are identical in every lane of the group, so the
+// group's control flow is uniform.
+//
+// Algorithm (restated lane by lane by the oracle's group mode):
+//  * propensities and criticality per reaction as in the one-lane kernel;
+//  * a0 and every cumulative selection use the lane partial sums (each
+//    lane's reactions in ascending order) combined by a Hillis-Steele
+//    inclusive scan over the G lanes; a selection takes the first lane whose
+//    inclusive sum reaches the threshold, then walks that lane's reactions;
+//  * tau' per species as in the one-lane kernel, minimum across lanes;
+//  * group-wide draws (tau'', critical pick, SSA uniforms) come from the
+//    event's Philox stream with counter word 3 = 0, lane l's Poisson counts
+//    from the stream with word 3 = 1 + l;
+//  * the leap update and its negativity / int32 test run per species over
+//    the species-major fear entries in ascending reaction order, which
+//    rejects exactly when the reaction-major order of test_update
+//    (2phase_n-tau-leaping.cu:1300-1351) would.
+#pragma once
+
+#include "ctl_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace ctl {
+
+// LDS ordering between lanes of one wavefront (a group never spans two).
+__device__ __forceinline__ void gsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int G>
+__device__ __forceinline__ uint64_t gballot(bool pred) {
+    const uint64_t b = __ballot(pred);
+    if (G == 64) return b;
+    const int base = (int)(threadIdx.x & 63u) & ~(G - 1);
+    return (b >> base) & ((1ull << G) - 1ull);
+}
+
+// Hillis-Steele inclusive scan over the group: v_l <- v_{l-d} + v_l.
+template <int G>
+__device__ __forceinline__ float gscan(float v, int l) {
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+        const float t = __shfl_up(v, d, G);
+        if (l >= d) v = t + v;
+    }
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ float gmin(float v) {
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) {
+        const float o = __shfl_xor(v, m, G);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ uint64_t gsum64(uint64_t v) {
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, m, G);
+        const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), m, G);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+// Group state: the scalars are identical in every lane of the group; the
+// Poisson / gap counters are per lane and summed when the simulation ends.
+struct GLane {
+    int slot;                // group slot
+    int sim;                 // local index, -1 none, -2 queue exhausted
+    double t;
+    uint32_t cursor, evc, xb;
+    uint64_t firings, ssa_steps;
+    uint32_t leaps, rejections, switches, absorbed;
+    uint64_t poisson, gaps;
+    Rng gg;                  // group stream (word 3 = 0)
+    Rng lg;                  // this lane's stream (word 3 = 1 + lane)
+};
+
+struct GView {
+    int32_t* Xb;             // [2][S]
+    uint32_t* A;             // [R]
+    int32_t* Kc;             // [R]
+    float* Cs;               // [R]
+};
+
+// Propensity of reaction j from the group's LDS state; the arithmetic of
+// prop_one (CalculatePropensity, T.cu:1443-1470, in FP32).
+__device__ __forceinline__ float gprop(const KParams& p, const int32_t* X, const float* Cs, int j, bool& crit) {
+    float a = Cs[j];
+    bool c = false;
+    const uint32_t nc = (uint32_t)p.nc;
+    const uint32_t h1 = p.Lptr[j + 1];
+    for (uint32_t h = p.Lptr[j]; h < h1; ++h) {
+        const uint32_t e = p.Lent[h];
+        const int s = (int)(e & 0xffffu);
+        const int v = (int)(e >> 16);
+        const int32_t x = X[s];
+        if (v == 1) {
+            a = a * (float)x;
+        } else if (v == 2) {
+            a = a * ((x < 2) ? 0.0f : (0.5f * (float)x) * (float)(x - 1));
+        } else if (x < v) {
+            a = a * 0.0f;
+        } else {
+            for (int q = 1; q <= v; ++q) a = a * ((float)(x - q + 1) / (float)q);
+        }
+        const uint32_t fl = (uint32_t)x / (uint32_t)v;
+        c = c || (fl > 0u && fl < nc);
+    }
+    crit = c;
+    return a;
+}
+
+// First lane whose inclusive sum P reaches w, then the walk through that
+// lane's reactions (which = 0: all; which = 2: critical ones plus reaction 0,
+// the reference's pick walk that starts from a_0, T.cu:1993-2006).  Returns
+// the chosen reaction (group-uniform) or -1.
+template <int G>
+__device__ __forceinline__ int gselect(const KParams& p, const uint32_t* A, float P, float w, int l, int which) {
+    const uint64_t b = gballot<G>(P >= w);
+    if (b == 0) return -1;
+    const int sel = __builtin_ctzll(b);
+    const float excl = __shfl_up(P, 1, G);
+    int pick = -1;
+    if (l == sel) {
+        float acc = l > 0 ? excl : 0.0f;
+        int last = -1;
+        for (int j = l; j < p.R; j += G) {
+            const uint32_t u = A[j];
+            if (which == 2 && !((u & 0x80000000u) || j == 0)) continue;
+            const float a = aval(u);
+            acc = acc + a;
+            if (a > 0.0f) last = j;
+            if (w <= acc) { pick = j; break; }
+        }
+        if (pick < 0) pick = last;
+    }
+    return __shfl(pick, sel, G);
+}
+
+template <int G>
+__device__ __forceinline__ void gabsorb(const KParams& p, GLane& L, const int32_t* X, int l) {
+    uint32_t* out = p.storage + (size_t)L.sim * p.K * p.SS;
+    if (L.cursor == 0) {
+        for (int sk = l; sk < p.SS; sk += G) out[sk] = (uint32_t)X[p.cs[sk]];
+        L.cursor = 1;
+    }
+    const uint32_t* last = out + (size_t)(L.cursor - 1) * p.SS;
+    for (int i = (int)L.cursor; i < p.K; ++i)
+        for (int sk = l; sk < p.SS; sk += G) out[(size_t)i * p.SS + sk] = last[sk];
+    L.t = p.tmax + (double)0.1f;
+    L.cursor = (uint32_t)p.K;
+    L.absorbed = 1;
+}
+
+// One adaptive leap attempt (PreCalculations, T.cu:1568-2093).
+template <int G>
+__device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V, int l,
+                                     const float* __restrict__ recip) {
+    if (L.t >= p.tmax) return LEAP_DONE;
+    int32_t* X = V.Xb + (L.xb ? p.S : 0);
+    int32_t* X1 = V.Xb + (L.xb ? 0 : p.S);
+    float P = 0.0f, PC = 0.0f;
+    for (int j = l; j < p.R; j += G) {
+        bool crit;
+        const float a = gprop(p, X, V.Cs, j, crit);
+        V.A[j] = __float_as_uint(a) | (crit ? 0x80000000u : 0u);
+        P = P + a;
+        if (crit) PC = PC + a;
+    }
+    gsync();
+    P = gscan<G>(P, l);
+    PC = gscan<G>(PC, l);
+    const float a0 = __shfl(P, G - 1, G);
+    const float a0c = __shfl(PC, G - 1, G);
+    if (a0 == 0.0f) { gabsorb<G>(p, L, X, l); return LEAP_ABSORB; }
+    // tau' (T.cu:1860-1922): the one-lane kernel's per-species values, min over lanes
+    float tau1 = 3.40282347e+38f;
+    for (int sp = l; sp < p.S; sp += G) {
+        const uint32_t h0 = p.Vptr[sp], h1 = p.Vptr[sp + 1];
+        if (h0 == h1) continue;
+        float mu = 0.0f, sg = 0.0f;
+        for (uint32_t h = h0; h < h1; ++h) {
+            const uint32_t e = p.Vent[h];
+            const int j = (int)(e & 0xffffu);
+            const int v = (int)(int16_t)(e >> 16);
+            const uint32_t u = V.A[j];
+            if (u & 0x80000000u) continue;
+            const float a = __uint_as_float(u);
+            mu = mu + (float)v * a;
+            sg = sg + (float)(v * v) * a;
+        }
+        if (mu == 0.0f || sg == 0.0f) continue;
+        const int32_t x = X[sp];
+        const float gg = gfun(p.horw[sp], x);
+        const float tmp2 = (p.eps * (float)x) / gg;
+        const float macs = tmp2 > 1.0f ? tmp2 : 1.0f;
+        const float t3 = macs / fabsf(mu);
+        const float t4 = (macs * macs) / sg;
+        const float tmp = t3 < t4 ? t3 : t4;
+        if (tmp < tau1) tau1 = tmp;
+    }
+    tau1 = gmin<G>(tau1);
+    if (tau1 < p.ssa_factor / a0) { L.switches++; return LEAP_SWITCH; }      // T.cu:1935-1941
+    uint32_t e = L.evc++;
+    rng_event(L.gg, e);
+    float tau2 = 3.40282347e+38f;
+    if (a0c > 0.0f) tau2 = exp_rv(L.gg) / a0c;                                // T.cu:1962-1965
+    float tau = tau1 < tau2 ? tau1 : tau2;
+    int pick = -1;
+    if (tau1 > tau2) {                                                         // T.cu:1993-2006
+        const float w = rng_oc(L.gg) * a0c;
+        float Q = 0.0f;
+        for (int j = l; j < p.R; j += G) {
+            const uint32_t u = V.A[j];
+            if ((u & 0x80000000u) || j == 0) Q = Q + aval(u);
+        }
+        Q = gscan<G>(Q, l);
+        pick = gselect<G>(p, V.A, Q, w, l, 2);
+    }
+    uint64_t fired = 0;
+    for (int attempt = 0;; ++attempt) {                                        // T.cu:2011-2050
+        if (attempt > 0) { e = L.evc++; L.rejections++; }
+        rng_event(L.lg, e);
+        uint64_t f = 0;
+        double cum = 0.0, pos = -1.0;
+        for (int j = l; j < p.R; j += G) {
+            const uint32_t u = V.A[j];
+            int32_t k;
+            if (u & 0x80000000u) {
+                k = (attempt == 0 && j == pick) ? 1 : 0;
+            } else {
+                const float lam = tau * __uint_as_float(u);
+                if (lam < 1.0f) {
+                    k = 0;
+                    if (lam > 0.0f) {
+                        if (pos < 0.0) { pos = (double)exp_rv(L.lg); L.gaps++; }
+                        cum = cum + (double)lam;
+                        while (pos < cum) { ++k; pos = pos + (double)exp_rv(L.lg); L.gaps++; }
+                    }
+                } else {
+                    k = poisson(L.lg, lam, recip);
+                    L.poisson++;
+                }
+            }
+            V.Kc[j] = k;
+            f += (uint64_t)k;
+        }
+        gsync();
+        bool bad = false;
+        for (int sp = l; sp < p.S; sp += G) {
+            int64_t x = X[sp];
+            const uint32_t h1 = p.FSptr[sp + 1];
+            for (uint32_t h = p.FSptr[sp]; h < h1; ++h) {
+                const uint32_t ent = p.FSent[h];
+                x += (int64_t)(int16_t)(ent >> 16) * (int64_t)V.Kc[ent & 0xffffu];
+                if (x < 0 || x > 2147483647LL) { bad = true; break; }
+            }
+            X1[sp] = (int32_t)x;
+        }
+        fired = gsum64<G>(f);
+        const bool any_bad = gballot<G>(bad) != 0;
+        gsync();
+        if (!any_bad) break;
+        tau = tau * 0.5f;
+    }
+    // check_time_instant (T.cu:1507-1562), per sampled species
+    {
+        uint32_t i = L.cursor;
+        const double tn = L.t + (double)tau;
+        if (i < (uint32_t)p.K && tn >= p.ts[i]) {
+            uint32_t* out = p.storage + (size_t)L.sim * p.K * p.SS;
+            while (i < (uint32_t)p.K && tn >= p.ts[i]) {
+                const double ti = p.ts[i];
+                for (int sk = l; sk < p.SS; sk += G) {
+                    const int sp = p.cs[sk];
+                    const int32_t d = X1[sp] - X[sp];
+                    uint32_t val = (uint32_t)X[sp];
+                    if (d > 0) {
+                        const double delta = ((double)d * (ti - L.t)) / (double)tau;
+                        val += (uint32_t)rint(delta);
+                    }
+                    out[(size_t)i * p.SS + sk] = val;
+                }
+                ++i;
+            }
+            L.cursor = i;
+        }
+    }
+    L.xb ^= 1u;
+    L.t = L.t + (double)tau;
+    L.firings += fired;
+    L.leaps++;
+    return LEAP_OK;
+}
+
+// One SSA burst (EventualSSASteps T.cu:2099-2273 / LightweightSSA
+// T.cu:2280-2454); after the fired reaction only its dependent propensities
+// are recomputed (same values as recomputing all).  Returns true on absorption.
+template <int G>
+__device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V, int l) {
+    int32_t* X = V.Xb + (L.xb ? p.S : 0);
+    rng_event(L.gg, L.evc++);
+    double tacc = 0.0;
+    for (int j = l; j < p.R; j += G) {
+        bool crit;
+        const float a = gprop(p, X, V.Cs, j, crit);
+        V.A[j] = __float_as_uint(a) | (crit ? 0x80000000u : 0u);
+    }
+    gsync();
+    for (int step = 0; step < p.ssa_steps; ++step) {
+        float P = 0.0f;
+        for (int j = l; j < p.R; j += G) P = P + aval(V.A[j]);
+        P = gscan<G>(P, l);
+        const float a0 = __shfl(P, G - 1, G);
+        if (a0 == 0.0f) { gabsorb<G>(p, L, X, l); return true; }
+        const float r1 = rng_oc(L.gg) * a0;
+        const int j = gselect<G>(p, V.A, P, r1, l, 0);
+        if (j < 0) { gabsorb<G>(p, L, X, l); return true; }    // unreachable for finite a0
+        const float dt = exp_rv(L.gg) / a0;
+        tacc = tacc + (double)dt;
+        const double tn = L.t + tacc;
+        uint32_t i = L.cursor;
+        if (i < (uint32_t)p.K && tn > p.ts[i]) {                               // T.cu:2213-2236
+            uint32_t* out = p.storage + (size_t)L.sim * p.K * p.SS;
+            while (i < (uint32_t)p.K && tn > p.ts[i]) {
+                for (int sk = l; sk < p.SS; sk += G) out[(size_t)i * p.SS + sk] = (uint32_t)X[p.cs[sk]];
+                ++i;
+            }
+            L.cursor = i;
+        }
+        gsync();
+        const uint32_t f1 = p.Fptr[j + 1];
+        for (uint32_t h = p.Fptr[j] + (uint32_t)l; h < f1; h += G) {
+            const uint32_t ent = p.Fent[h];
+            const int s = (int)(ent & 0xffffu);
+            const int v = (int)(int16_t)(ent >> 16);
+            X[s] = (int32_t)((uint32_t)X[s] + (uint32_t)v);
+        }
+        gsync();
+        L.ssa_steps++;
+        if (tn <= p.tmax) L.firings++;
+        const uint32_t d1 = p.Dptr[j + 1];
+        for (uint32_t d = p.Dptr[j] + (uint32_t)l; d < d1; d += G) {
+            const int r = (int)p.Dent[d];
+            bool crit;
+            const float a = gprop(p, X, V.Cs, r, crit);
+            V.A[r] = __float_as_uint(a) | (crit ? 0x80000000u : 0u);
+        }
+        gsync();
+    }
+    L.t = L.t + tacc;
+    return false;
+}
+
+template <int G>
+__device__ __forceinline__ void gload_rates(const KParams& p, const GView& V, int sim, int l) {
+    for (int j = l; j < p.R; j += G)
+        V.Cs[j] = p.c_bcast ? p.c_bcast[j] : p.c_sims[(size_t)(sim / p.c_div) * p.R + j];
+}
+
+// Next simulation from the queue (PrepareVariables, T.cu:1269-1297).
+template <int G>
+__device__ __forceinline__ bool gfetch(const KParams& p, GLane& L, const GView& V, int l) {
+    uint32_t q = 0;
+    if (l == 0) q = atomicAdd(p.next_sim, 1u);
+    q = __shfl(q, 0, G);
+    if (q >= (uint32_t)p.n) { L.sim = -2; return false; }
+    L.sim = (int)q;
+    L.t = 0.0;
+    L.cursor = 0;
+    L.evc = 0;
+    L.xb = 0;
+    L.firings = L.ssa_steps = 0;
+    L.leaps = L.rejections = L.switches = L.absorbed = 0;
+    L.poisson = L.gaps = 0;
+    L.gg.sim = L.lg.sim = p.sim_base + q;
+    L.gg.pos = L.lg.pos = 4;
+    for (int s = l; s < p.S; s += G)
+        V.Xb[s] = (int32_t)(p.x0_bcast ? p.x0_bcast[s] : p.x0_sims[(size_t)(q / (uint32_t)p.x0_div) * p.S + s]);
+    gload_rates<G>(p, V, (int)q, l);
+    gsync();
+    return true;
+}
+
+template <int G>
+__device__ __forceinline__ void gfinish(const KParams& p, GLane& L, const GView& V, int l) {
+    const int32_t* X = V.Xb + (L.xb ? p.S : 0);
+    const uint64_t pois = gsum64<G>(L.poisson);
+    const uint64_t gaps = gsum64<G>(L.gaps);
+    for (int s = l; s < p.S; s += G) p.xfinal[(size_t)L.sim * p.S + s] = X[s];
+    if (l == 0) {
+        ctl_sim_stat st;
+        st.firings = L.firings;
+        st.ssa_steps = L.ssa_steps;
+        st.poisson = pois;
+        st.gaps = gaps;
+        st.t_end = L.t;
+        st.leaps = L.leaps;
+        st.rejections = L.rejections;
+        st.switches = L.switches;
+        st.events = L.evc;
+        st.absorbed = L.absorbed;
+        st.reserved = 0;
+        p.simstat[L.sim] = st;
+        p.tfinal[L.sim] = L.t;
+        atomicAdd(p.done, 1u);
+    }
+    L.sim = -1;
+}
+
+// Persistent group kernel: p.ns lanes = p.ns / G group slots, each pulling
+// simulations from the queue; a launch runs up to p.iters leap attempts (each
+// with its SSA burst when it switched) per group, then parks the group's
+// state in the slot arrays for the next launch.
+template <int G>
+__global__ __launch_bounds__(256) void group_kernel(KParams p) {
+    extern __shared__ uint32_t smem[];
+    __shared__ float recip[64];
+    for (int i = threadIdx.x; i < 64; i += blockDim.x) recip[i] = c_recip[i];
+    __syncthreads();
+    const int l = (int)(threadIdx.x & (unsigned)(G - 1));
+    const int gib = (int)(threadIdx.x / G);
+    GLane L;
+    L.slot = (int)blockIdx.x * (int)(blockDim.x / G) + gib;
+    if (L.slot * G >= p.ns) return;
+    const size_t words = (size_t)2 * p.S + (size_t)3 * p.R;
+    GView V;
+    V.Xb = (int32_t*)(smem + (size_t)gib * words);
+    V.A = (uint32_t*)(V.Xb + 2 * p.S);
+    V.Kc = (int32_t*)(V.A + p.R);
+    V.Cs = (float*)(V.Kc + p.R);
+    L.gg.k0 = L.lg.k0 = p.key0;
+    L.gg.k1 = L.lg.k1 = p.key1;
+    L.gg.tag = 0u;
+    L.lg.tag = 1u + (uint32_t)l;
+    L.sim = p.st.sim[L.slot];
+    if (L.sim == -2) return;
+    if (L.sim >= 0) {                    // resume a parked simulation
+        const int s0 = L.slot;
+        L.t = p.st.t[s0];
+        L.cursor = p.st.cursor[s0];
+        L.evc = p.st.evc[s0];
+        L.xb = 0;
+        L.firings = p.st.firings[s0];
+        L.ssa_steps = p.st.ssa_steps[s0];
+        L.leaps = p.st.leaps[s0];
+        L.rejections = p.st.rejections[s0];
+        L.switches = p.st.switches[s0];
+        L.absorbed = 0;
+        L.poisson = p.st.poisson[(size_t)s0 * G + l];
+        L.gaps = p.st.gaps[(size_t)s0 * G + l];
+        L.gg.sim = L.lg.sim = p.sim_base + (uint32_t)L.sim;
+        L.gg.pos = L.lg.pos = 4;
+        for (int s = l; s < p.S; s += G) V.Xb[s] = p.X[(size_t)s0 * p.S + s];
+        gload_rates<G>(p, V, L.sim, l);
+        gsync();
+    }
+    for (int it = 0; it < p.iters; ++it) {
+        if (L.sim < 0 && !gfetch<G>(p, L, V, l)) break;
+        if (p.force_ssa) {
+            if (L.t >= p.tmax || gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);
+            continue;
+        }
+        const int r = gleap<G>(p, L, V, l, recip);
+        if (r == LEAP_DONE || r == LEAP_ABSORB) {
+            gfinish<G>(p, L, V, l);
+        } else if (r == LEAP_SWITCH) {
+            if (gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);
+        }
+    }
+    const int s0 = L.slot;
+    if (L.sim >= 0) {                    // park: state to the slot arrays
+        const int32_t* X = V.Xb + (L.xb ? p.S : 0);
+        for (int s = l; s < p.S; s += G) p.X[(size_t)s0 * p.S + s] = X[s];
+        p.st.poisson[(size_t)s0 * G + l] = L.poisson;
+        p.st.gaps[(size_t)s0 * G + l] = L.gaps;
+        if (l == 0) {
+            p.st.t[s0] = L.t;
+            p.st.cursor[s0] = L.cursor;
+            p.st.evc[s0] = L.evc;
+            p.st.firings[s0] = L.firings;
+            p.st.ssa_steps[s0] = L.ssa_steps;
+            p.st.leaps[s0] = L.leaps;
+            p.st.rejections[s0] = L.rejections;
+            p.st.switches[s0] = L.switches;
+        }
+    }
+    if (l == 0) p.st.sim[s0] = L.sim;
+}
+
+}  // namespace ctl

@@ -59,6 +59,8 @@ struct KParams {
     const uint32_t* Vent;
     const uint32_t* Dptr;     // SSA dependency graph: reactions whose propensity
     const uint32_t* Dent;     //   reads a species changed by reaction j
+    const uint32_t* FSptr;    // fear entries species-major, ascending reaction
+    const uint32_t* FSent;    //   (group kernel's per-species update)
     const uint32_t* horw;     // hor | hort << 8
     const int32_t* cs;
     const double* ts;
@@ -439,7 +441,7 @@ __device__ __forceinline__ void load_lane(const KParams& p, Lane& L) {
     L.g.blk = p.st.blk[s];
     L.g.pos = p.st.pos[s];
     if (L.g.pos < 4u) {   // regenerate the partially consumed block of an SSA burst
-        philox_block(L.g.blk - 1u, L.g.ev, L.g.sim, 0u, L.g.k0, L.g.k1, L.g.b0, L.g.b1, L.g.b2, L.g.b3);
+        philox_block(L.g.blk - 1u, L.g.ev, L.g.sim, L.g.tag, L.g.k0, L.g.k1, L.g.b0, L.g.b1, L.g.b2, L.g.b3);
         for (uint32_t i = 0; i < L.g.pos; ++i) {   // drop the words already consumed
             L.g.b0 = L.g.b1;
             L.g.b1 = L.g.b2;

@@ -27,8 +27,9 @@
 #include <mutex>
 #include <vector>
 
-#include "ctl_kernels.h"
+#include "ctl_group.h"
 #include "ctl_internal.h"
+#include "ctl_kernels.h"
 
 #pragma clang fp contract(off)
 
@@ -109,6 +110,18 @@ __global__ __launch_bounds__(256) void tau_leap_kernel(KParams p) {
     store_lane(p, L);
 }
 
+const void* group_kernel_fn(int G) {
+    switch (G) {
+        case 2: return (const void*)group_kernel<2>;
+        case 4: return (const void*)group_kernel<4>;
+        case 8: return (const void*)group_kernel<8>;
+        case 16: return (const void*)group_kernel<16>;
+        case 32: return (const void*)group_kernel<32>;
+        case 64: return (const void*)group_kernel<64>;
+    }
+    return nullptr;
+}
+
 // Reference sample layout [i][sk][sim] from the kernel's [sim][i][sk].
 __global__ void to_reference_layout(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int n, int KS) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -186,6 +199,7 @@ struct ctl_sim {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int n = 0, ns = 0, block = 256, grid = 0;
+    int lanes = 1;                   // lanes per simulation (1: tau_leap_kernel, else group_kernel<lanes>)
     bool lds = false;
     size_t smem = 0;
     KParams kp{};
@@ -299,25 +313,59 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     e = hipGetDeviceProperties(&prop, s->device);
     if (e != hipSuccess) return bail(hip_fail(e, "hipGetDeviceProperties"));
     const int S = m.S, R = m.R, K = (int)m.ts.size(), SS = (int)m.cs.size();
-    // Launch geometry.  Per-lane working set (state, trial state,
-    // propensities, per-simulation rates) in LDS when a 64-lane wavefront's
-    // share stays within 16 KB (>= 8 wavefronts per CU); otherwise in the
-    // global slot arrays with 256-lane workgroups.
+    // Launch geometry.  One lane per simulation when its working set (state,
+    // trial state, propensities, per-simulation rates) fits a 64-lane
+    // wavefront's 16 KB share of LDS, or when asked for; otherwise G lanes
+    // per simulation with the simulation's state in LDS (group kernel).
     const size_t lane_words = (size_t)2 * S + R + (m.c_rows == 1 ? 0 : R);
     const bool lds_auto = lane_words * 4 * 64 <= 16384;
-    s->lds = o->block_size < 0 ? false : lds_auto;
-    s->block = o->block_size > 0 ? o->block_size : (s->lds ? 64 : 256);
-    if (s->block % 64 != 0 || s->block > 256) return bail(ctl::fail(CTL_E_ARGS, "block_size must be 64..256, multiple of 64"));
-    s->smem = s->lds ? lane_words * 4 * s->block : 0;
-    if (s->smem > 65536) { s->lds = false; s->smem = 0; }
-    const void* kfn = s->lds ? (const void*)tau_leap_kernel<true> : (const void*)tau_leap_kernel<false>;
+    const size_t gwords = (size_t)2 * S + (size_t)3 * R;     // group LDS words per simulation
+    int G = o->lanes_per_sim;
+    if (G == 0) {
+        if (lds_auto || gwords * 4 > 65536) {
+            G = 1;
+        } else {
+            G = 4;
+            while (G < 64 && G * 16 < R) G *= 2;      // about 16 reactions per lane at most
+        }
+    }
+    if (G < 1 || G > 64 || (G & (G - 1)))
+        return bail(ctl::fail(CTL_E_ARGS, "lanes_per_sim must be 0 or a power of two up to 64"));
+    s->lanes = G;
     int per_cu = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, s->block, s->smem);
-    if (e != hipSuccess || per_cu <= 0) per_cu = 1;
-    int ns = o->slots > 0 ? o->slots : prop.multiProcessorCount * per_cu * s->block;
-    if (ns > n) ns = ((n + s->block - 1) / s->block) * s->block;
-    s->ns = ns;
-    s->grid = (ns + s->block - 1) / s->block;
+    if (G > 1) {
+        if (gwords * 4 > 65536) return bail(ctl::fail(CTL_E_LIMIT, "model too large for the group kernel (2S+3R > 16384)"));
+        int gpb = 256 / G;                                    // groups per workgroup
+        while (gpb > 64 / G && (size_t)gpb * gwords * 4 > 65536) gpb /= 2;
+        if ((size_t)gpb * gwords * 4 > 65536) gpb = 64 / G;   // one wavefront
+        if ((size_t)gpb * gwords * 4 > 65536)
+            return bail(ctl::fail(CTL_E_LIMIT, "model too large for this lanes_per_sim (use 64)"));
+        s->block = gpb * G;
+        s->smem = (size_t)gpb * gwords * 4;
+        s->lds = true;
+        const void* kfn = group_kernel_fn(G);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, s->block, s->smem);
+        if (e != hipSuccess || per_cu <= 0) per_cu = 1;
+        long long groups = o->slots > 0 ? (long long)o->slots : (long long)prop.multiProcessorCount * per_cu * gpb;
+        if (groups > n) groups = ((n + gpb - 1) / gpb) * (long long)gpb;
+        groups = ((groups + gpb - 1) / gpb) * gpb;
+        s->ns = (int)(groups * G);
+        s->grid = (int)(groups / gpb);
+    } else {
+        s->lds = o->block_size < 0 ? false : lds_auto;
+        s->block = o->block_size > 0 ? o->block_size : (s->lds ? 64 : 256);
+        if (s->block % 64 != 0 || s->block > 256) return bail(ctl::fail(CTL_E_ARGS, "block_size must be 64..256, multiple of 64"));
+        s->smem = s->lds ? lane_words * 4 * s->block : 0;
+        if (s->smem > 65536) { s->lds = false; s->smem = 0; }
+        const void* kfn = s->lds ? (const void*)tau_leap_kernel<true> : (const void*)tau_leap_kernel<false>;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, s->block, s->smem);
+        if (e != hipSuccess || per_cu <= 0) per_cu = 1;
+        int ns = o->slots > 0 ? o->slots : prop.multiProcessorCount * per_cu * s->block;
+        if (ns > n) ns = ((n + s->block - 1) / s->block) * s->block;
+        s->ns = ns;
+        s->grid = (ns + s->block - 1) / s->block;
+    }
+    const int ns = s->ns;
 
     KParams& p = s->kp;
     p.S = S; p.R = R; p.K = K; p.SS = SS;
@@ -346,7 +394,20 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
         }
         Dptr[R] = (uint32_t)Dent.size();
     }
-    uint32_t *Lptr, *Lent, *Fptr, *Fent, *Vptr, *Vent, *Dp, *De, *hw;
+    // fear entries species-major, ascending reaction (group kernel update)
+    std::vector<uint32_t> FSptr(S + 1, 0), FSent(m.Fent.size());
+    {
+        std::vector<uint32_t> cnt(S + 1, 0);
+        for (uint32_t e : m.Fent) cnt[(e & 0xffffu) + 1]++;
+        for (int i = 0; i < S; ++i) cnt[i + 1] += cnt[i];
+        for (int i = 0; i <= S; ++i) FSptr[i] = cnt[i];
+        for (int j = 0; j < R; ++j)
+            for (uint32_t h = m.Fptr[j]; h < m.Fptr[j + 1]; ++h) {
+                const uint32_t e = m.Fent[h];
+                FSent[cnt[e & 0xffffu]++] = (uint32_t)j | (e & 0xffff0000u);
+            }
+    }
+    uint32_t *Lptr, *Lent, *Fptr, *Fent, *Vptr, *Vent, *Dp, *De, *hw, *FSp, *FSe;
     int32_t* cs;
     double* ts;
     if ((rc = upload(s, &Lptr, m.Lptr.data(), m.Lptr.size())) || (rc = upload(s, &Lent, m.Lent.data(), m.Lent.size())) ||
@@ -354,8 +415,11 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
         (rc = upload(s, &Vptr, m.Vptr.data(), m.Vptr.size())) || (rc = upload(s, &Vent, m.Vent.data(), m.Vent.size())) ||
         (rc = upload(s, &Dp, Dptr.data(), Dptr.size())) || (rc = upload(s, &De, Dent.data(), Dent.size())) ||
         (rc = upload(s, &hw, horw.data(), horw.size())) || (rc = upload(s, &cs, m.cs.data(), m.cs.size())) ||
-        (rc = upload(s, &ts, m.ts.data(), m.ts.size())))
+        (rc = upload(s, &ts, m.ts.data(), m.ts.size())) ||
+        (rc = upload(s, &FSp, FSptr.data(), FSptr.size())) || (rc = upload(s, &FSe, FSent.data(), FSent.size())))
         return bail(rc);
+    p.FSptr = FSp;
+    p.FSent = FSe;
     p.Lptr = Lptr; p.Lent = Lent; p.Fptr = Fptr; p.Fent = Fent; p.Vptr = Vptr; p.Vent = Vent;
     p.Dptr = Dp; p.Dent = De;
     p.horw = hw; p.cs = cs; p.ts = ts; p.tmax = m.ts.back();
@@ -379,11 +443,15 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     p.ssa_factor = o->ssa_factor;
     p.force_ssa = o->force_ssa ? 1 : 0;
     p.ssa_chunk = o->ssa_chunk > 0 ? o->ssa_chunk : 10;
-    p.iters = o->iters_per_launch > 0 ? o->iters_per_launch : 4096;
+    p.iters = o->iters_per_launch > 0 ? o->iters_per_launch : (G > 1 ? 65536 : 4096);
     p.ns = ns;
-    if ((rc = dalloc(s, &p.X, (size_t)2 * S * ns)) || (rc = dalloc(s, &p.A, (size_t)R * ns))) return bail(rc);
     p.Cs = nullptr;
-    if (!p.c_bcast && (rc = dalloc(s, &p.Cs, (size_t)R * ns))) return bail(rc);
+    if (G > 1) {                         // parked group state only: X [group][S]
+        if ((rc = dalloc(s, &p.X, (size_t)S * (ns / G))) || (rc = dalloc(s, &p.A, 1))) return bail(rc);
+    } else {
+        if ((rc = dalloc(s, &p.X, (size_t)2 * S * ns)) || (rc = dalloc(s, &p.A, (size_t)R * ns))) return bail(rc);
+        if (!p.c_bcast && (rc = dalloc(s, &p.Cs, (size_t)R * ns))) return bail(rc);
+    }
     SlotState& st = p.st;
     if ((rc = dalloc(s, &st.sim, ns)) || (rc = dalloc(s, &st.t, ns)) || (rc = dalloc(s, &st.tacc, ns)) ||
         (rc = dalloc(s, &st.cursor, ns)) || (rc = dalloc(s, &st.mode, ns)) || (rc = dalloc(s, &st.evc, ns)) ||
@@ -401,6 +469,7 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     s->stats.slots = ns;
     s->stats.block_size = s->block;
     s->stats.lds_state = s->lds ? 1 : 0;
+    s->stats.lanes_per_sim = G;
     *out = s;
     return 0;
 }
@@ -423,10 +492,21 @@ int ctl_sim_run(ctl_sim* s) {
     float total_ms = 0.0f;
     while (done < (uint32_t)s->n) {
         HIPCHK(hipEventRecord(s->ev0, s->stream), "hipEventRecord");
-        if (s->lds)
-            hipLaunchKernelGGL(tau_leap_kernel<true>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p);
-        else
-            hipLaunchKernelGGL(tau_leap_kernel<false>, dim3(s->grid), dim3(s->block), 0, s->stream, p);
+        switch (s->lanes) {
+            case 1:
+                if (s->lds)
+                    hipLaunchKernelGGL(tau_leap_kernel<true>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p);
+                else
+                    hipLaunchKernelGGL(tau_leap_kernel<false>, dim3(s->grid), dim3(s->block), 0, s->stream, p);
+                break;
+            case 2: hipLaunchKernelGGL(group_kernel<2>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
+            case 4: hipLaunchKernelGGL(group_kernel<4>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
+            case 8: hipLaunchKernelGGL(group_kernel<8>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
+            case 16: hipLaunchKernelGGL(group_kernel<16>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
+            case 32: hipLaunchKernelGGL(group_kernel<32>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
+            case 64: hipLaunchKernelGGL(group_kernel<64>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
+            default: return ctl::fail(CTL_E_STATE, "bad lanes_per_sim");
+        }
         HIPCHK(hipGetLastError(), "tau_leap_kernel launch");
         HIPCHK(hipEventRecord(s->ev1, s->stream), "hipEventRecord");
         HIPCHK(hipMemcpyAsync(&done, p.done, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream), "hipMemcpy D2H");

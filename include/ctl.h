@@ -80,6 +80,9 @@ typedef struct {
     int32_t  slots;            /* resident simulation slots */
     int32_t  iters_per_launch; /* leap/SSA iterations per slot per launch */
     int32_t  ssa_chunk;        /* SSA steps per iteration of a slot */
+    int32_t  lanes_per_sim;    /* 1: one lane runs one simulation; 2..64 (power of two): that many
+                                  lanes of a wavefront cooperate on one simulation whose state
+                                  stays in LDS (the group algorithm); 0: chosen by the library */
 } ctl_opts;
 
 typedef struct {
@@ -98,6 +101,7 @@ typedef struct {
     int32_t  slots;            /* resident slots used */
     int32_t  block_size;
     int32_t  lds_state;        /* 1 when per-lane state lives in LDS */
+    int32_t  lanes_per_sim;    /* lanes per simulation used */
 } ctl_stats;
 
 typedef struct {

@@ -1,0 +1,161 @@
+"""GPU parity of the group kernel (lanes_per_sim = G > 1: G lanes cooperate
+on one simulation, its state in LDS) through the C ABI against the oracle's
+group algorithm with the same G: integer samples, final states and every
+per-simulation counter bit-exact, plus statistics and properties at size."""
+import os
+
+import numpy as np
+import pytest
+
+import stats_util as su
+from conftest import GOLDEN
+from test_gpu_parity import SCH, SCH_TS, _compare, _random_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("G,force", [(2, False), (4, False), (4, True)])
+def test_group_schlogl_bit_exact(gpu, oracle_lib, G, force):
+    print(f"case: group schlogl G={G} force_ssa={force}", flush=True)
+    m = gpu.Model.from_folder(os.path.join(GOLDEN, "schlogl"), 0, 1024)
+    with gpu.Sim(m, 64, 16, seed=77, force_ssa=force, lanes_per_sim=G) as s:
+        s.run()
+        assert s.stats()["lanes_per_sim"] == G
+        out, st, xf = oracle_lib.run(SCH["left"], SCH["right"], SCH["x0"], SCH["c"], SCH_TS, locked=SCH["feed"],
+                                     n=1024, seed=77, force_ssa=force, final=True, group=G)
+        _compare(s, out, st, xf)
+
+
+@pytest.mark.parametrize("seed,N,M,n,per_sim,offset,G,geom", [
+    (11, 6, 12, 300, True, 0, 4, {}),
+    (12, 12, 30, 65, True, 17, 8, {"iters_per_launch": 5}),
+    (13, 20, 40, 1, False, 1000, 16, {}),
+    (14, 9, 9, 257, False, 3, 2, {"slots": 24, "iters_per_launch": 9}),
+    (15, 30, 70, 129, True, 5, 32, {"iters_per_launch": 17}),
+    (16, 40, 90, 77, False, 0, 64, {}),
+])
+def test_group_random_networks_bit_exact(gpu, oracle_lib, seed, N, M, n, per_sim, offset, G, geom):
+    """Per-simulation rates and initial states, feed locks, sampled-species
+    subsets, ragged sizes, every group width, input offsets, and small
+    iteration budgets / slot counts that park simulations across launches."""
+    print(f"case: group random seed={seed} N={N} M={M} n={n} G={G} {geom}", flush=True)
+    net, feed, x0, c, cs, ts = _random_case(seed, N, M, n, per_sim)
+    m = gpu.Model.from_arrays(net["left"], net["right"], x0, c, ts, feed=feed, cs=cs)
+    with gpu.Sim(m, n, 1, seed=seed * 1000 + 1, offset=offset, lanes_per_sim=G, **geom) as s:
+        s.run()
+        out, st, xf = oracle_lib.run(net["left"], net["right"], x0, c, ts, locked=feed, cs=cs, n=n,
+                                     seed=seed * 1000 + 1, sim_base=offset, final=True, group=G)
+        _compare(s, out, st, xf)
+
+
+def test_group_high_order_and_absorbing_bit_exact(gpu, oracle_lib):
+    print("case: group high order + absorbing", flush=True)
+    left = [[3, 0, 0], [0, 2, 0], [1, 0, 0]]
+    right = [[0, 0, 0], [0, 0, 1], [0, 0, 0]]
+    x0 = np.array([[60, 40, 0], [5, 3, 0], [500, 200, 0], [2, 1, 7]], np.uint32).repeat(64, axis=0)
+    c = [[0.01, 0.05, 0.2]]
+    ts = np.linspace(0.05, 12.0, 60)
+    for force in (False, True):
+        m = gpu.Model.from_arrays(left, right, x0, c, ts)
+        with gpu.Sim(m, 256, 1, seed=5, force_ssa=force, lanes_per_sim=2) as s:
+            s.run()
+            out, st, xf = oracle_lib.run(left, right, x0, c, ts, n=256, seed=5, force_ssa=force, final=True,
+                                         group=2)
+            assert st["absorbed"].sum() > 0
+            _compare(s, out, st, xf)
+
+
+@pytest.mark.parametrize("G", [16, 64])
+def test_group_mass_action_100x200_subset_bit_exact(gpu, oracle_lib, G):
+    """The benchmark network on 24 simulations at both ends of 2^20."""
+    from cutauleaping_amd import networks
+    print(f"case: group 100x200 network G={G}", flush=True)
+    net = networks.mass_action(100, 200, seed=1)
+    ts = networks.sample_times(200, 50.0)
+    m = gpu.Model.from_arrays(net["left"], net["right"], net["x0"], net["c"], ts)
+    for offset in (0, (1 << 20) - 24):
+        with gpu.Sim(m, 24, 1, seed=0xC0FFEE, offset=offset, lanes_per_sim=G) as s:
+            s.run()
+            out, st, xf = oracle_lib.run(net["left"], net["right"], net["x0"], net["c"], ts, n=24, seed=0xC0FFEE,
+                                         sim_base=offset, final=True, group=G)
+            _compare(s, out, st, xf)
+
+
+@pytest.mark.parametrize("force", [False, True])
+def test_group_enzyme_cascade_bit_exact(gpu, oracle_lib, force):
+    from cutauleaping_amd import networks
+    print(f"case: group enzyme cascade force_ssa={force}", flush=True)
+    net = networks.enzyme_cascade(20, seed=4)
+    ts = networks.sample_times(100, 100.0)
+    m = gpu.Model.from_arrays(net["left"], net["right"], net["x0"], net["c"], ts)
+    with gpu.Sim(m, 128, 1, seed=31, force_ssa=force, lanes_per_sim=4) as s:
+        s.run()
+        out, st, xf = oracle_lib.run(net["left"], net["right"], net["x0"], net["c"], ts, n=128, seed=31,
+                                     force_ssa=force, final=True, group=4)
+        _compare(s, out, st, xf)
+    n = 20
+    assert np.all(xf[:, :n] + xf[:, 2 * n + 1:] == net["x0"][:n] + net["x0"][2 * n + 1:])
+
+
+def test_group_replica_group_rows_bit_exact(gpu, oracle_lib):
+    from cutauleaping_amd import networks
+    print("case: group replica-group rows", flush=True)
+    net = networks.mass_action(10, 20, seed=8, t_end=2.0)
+    rng = np.random.default_rng(8)
+    c = net["c"][None, :] * 10 ** rng.uniform(-1, 1, (16, 20))
+    x0 = (net["x0"][None, :] * rng.uniform(0.5, 1.5, (16, 10))).astype(np.uint32)
+    ts = np.linspace(0.1, 2.0, 20)
+    m = gpu.Model.from_arrays(net["left"], net["right"], x0, c, ts)
+    with gpu.Sim(m, 32, 16, seed=12, lanes_per_sim=8) as s:
+        s.run()
+        out, st = oracle_lib.run(net["left"], net["right"], x0, c, ts, n=512, seed=12, group=8)
+        assert np.array_equal(s.samples(), out)
+        mean, sd = s.moments(32)
+    ref = out.reshape(16, 32, 20, 10).astype(float)
+    assert np.allclose(mean, ref.mean(axis=1)) and np.allclose(sd, ref.std(axis=1), atol=1e-6)
+
+
+def test_group_schlogl_statistics(gpu, oracle_lib):
+    """16384 group-kernel simulations vs the exact master equation and vs the
+    reference-type restatement."""
+    print("case: group schlogl statistics 16384", flush=True)
+    cme = np.load(os.path.join(GOLDEN, "schlogl_cme.npz"))
+    m = gpu.Model.from_arrays(SCH["left"], SCH["right"], SCH["x0"], SCH["c"], SCH_TS, feed=SCH["feed"])
+    with gpu.Sim(m, 256, 64, seed=2025, lanes_per_sim=4) as s:
+        s.run()
+        X = s.samples()[:, :, 1]
+    n = X.shape[0]
+    for k, tc in enumerate(cme["t_cdf"]):
+        i = int(round(tc / 0.1)) - 1
+        assert su.ks_cdf_D(X[:, i], cme["cdf"][k]) < 1.63 / np.sqrt(n) * 1.2, tc
+    for i in (9, 49, 99):
+        assert su.mean_ci_ok(X[:, i], cme["mean"][i]) and su.var_ci_ok(X[:, i], cme["sd"][i] ** 2)
+    ref, _ = oracle_lib.run(SCH["left"], SCH["right"], SCH["x0"], SCH["c"], SCH_TS, locked=SCH["feed"],
+                            n=4096, seed=99, precision=oracle_lib.REFSEM)
+    ps = [su.ks_2samp_p(X[:, i], ref[:, i, 1]) for i in range(100)]
+    assert su.holm_ok(ps), min(ps)
+
+
+def test_group_mass_action_properties_65536(gpu, oracle_lib):
+    """2^16 simulations of the benchmark network with the library's own
+    choice of lanes per simulation: every simulation reaches t_end, counts
+    stay in int32, and 8 simulations across the batch equal the oracle."""
+    from cutauleaping_amd import networks
+    print("case: group 100x200 network, 65536 sims", flush=True)
+    net = networks.mass_action(100, 200, seed=1)
+    ts = networks.sample_times(200, 50.0)
+    m = gpu.Model.from_arrays(net["left"], net["right"], net["x0"], net["c"], ts)
+    with gpu.Sim(m, 256, 256, seed=0xC0FFEE) as s:
+        s.run()
+        G = s.stats()["lanes_per_sim"]
+        g = s.samples()
+        gs = s.sim_stats()
+        x, t = s.final_state()
+    assert G > 1
+    assert np.all(t >= 50.0) and np.all(x >= 0) and g.max() < 2 ** 31
+    assert np.all(gs["firings"] > 0)
+    for idx in np.linspace(0, 65535, 8).astype(int):
+        out, st = oracle_lib.run(net["left"], net["right"], net["x0"], net["c"], ts, n=1, seed=0xC0FFEE,
+                                 sim_base=int(idx), group=G)
+        assert np.array_equal(g[idx], out[0]), idx
+        assert gs["firings"][idx] == st["firings"][0]

@@ -1,4 +1,5 @@
-"""GPU parity: the gfx950 kernels through the C ABI against the CPU oracle.
+"""GPU parity of the one-lane-per-simulation kernel (lanes_per_sim=1)
+through the C ABI against the CPU oracle.
 
 Bit-exact (integer samples, final states and per-simulation counters) against
 the oracle's FP32 replay of the same Philox streams; statistical agreement
@@ -40,7 +41,7 @@ def test_schlogl_reference_config_bit_exact(gpu, oracle_lib, force):
     """Config 0: Schlogl, 1024 simulations (64 threads x 16 blocks) from the input folder."""
     print(f"case: schlogl 1024 force_ssa={force}", flush=True)
     m = gpu.Model.from_folder(os.path.join(GOLDEN, "schlogl"), 0, 1024)
-    with gpu.Sim(m, 64, 16, seed=77, force_ssa=force) as s:
+    with gpu.Sim(m, 64, 16, seed=77, force_ssa=force, lanes_per_sim=1) as s:
         s.run()
         out, st, xf = oracle_lib.run(SCH["left"], SCH["right"], SCH["x0"], SCH["c"], SCH_TS, locked=SCH["feed"],
                                      n=1024, seed=77, force_ssa=force, final=True)
@@ -75,7 +76,7 @@ def test_random_networks_bit_exact(gpu, oracle_lib, seed, N, M, n, per_sim, offs
     print(f"case: random seed={seed} N={N} M={M} n={n} per_sim={per_sim} offset={offset} {geom}", flush=True)
     net, feed, x0, c, cs, ts = _random_case(seed, N, M, n, per_sim)
     m = gpu.Model.from_arrays(net["left"], net["right"], x0, c, ts, feed=feed, cs=cs)
-    with gpu.Sim(m, n, 1, seed=seed * 1000 + 1, offset=offset, **geom) as s:
+    with gpu.Sim(m, n, 1, seed=seed * 1000 + 1, offset=offset, **geom, lanes_per_sim=1) as s:
         s.run()
         out, st, xf = oracle_lib.run(net["left"], net["right"], x0, c, ts, locked=feed, cs=cs, n=n,
                                      seed=seed * 1000 + 1, sim_base=offset, final=True)
@@ -93,7 +94,7 @@ def test_high_order_and_absorbing_bit_exact(gpu, oracle_lib):
     ts = np.linspace(0.05, 12.0, 60)
     for force in (False, True):
         m = gpu.Model.from_arrays(left, right, x0, c, ts)
-        with gpu.Sim(m, 256, 1, seed=5, force_ssa=force) as s:
+        with gpu.Sim(m, 256, 1, seed=5, force_ssa=force, lanes_per_sim=1) as s:
             s.run()
             out, st, xf = oracle_lib.run(left, right, x0, c, ts, n=256, seed=5, force_ssa=force, final=True)
             assert st["absorbed"].sum() > 0
@@ -109,7 +110,7 @@ def test_mass_action_100x200_subset_bit_exact(gpu, oracle_lib):
     ts = networks.sample_times(200, 50.0)
     m = gpu.Model.from_arrays(net["left"], net["right"], net["x0"], net["c"], ts)
     for offset in (0, (1 << 20) - 24):
-        with gpu.Sim(m, 24, 1, seed=0xC0FFEE, offset=offset) as s:
+        with gpu.Sim(m, 24, 1, seed=0xC0FFEE, offset=offset, lanes_per_sim=1) as s:
             s.run()
             out, st, xf = oracle_lib.run(net["left"], net["right"], net["x0"], net["c"], ts, n=24, seed=0xC0FFEE,
                                          sim_base=offset, final=True)
@@ -124,7 +125,7 @@ def test_schlogl_statistics(gpu, oracle_lib):
     cme = np.load(os.path.join(GOLDEN, "schlogl_cme.npz"))
     m = gpu.Model.from_arrays(SCH["left"], SCH["right"], SCH["x0"], SCH["c"], SCH_TS, feed=SCH["feed"])
     for force in (False, True):
-        with gpu.Sim(m, 256, 64, seed=2024 + force, force_ssa=force) as s:
+        with gpu.Sim(m, 256, 64, seed=2024 + force, force_ssa=force, lanes_per_sim=1) as s:
             s.run()
             X = s.samples()[:, :, 1]
         n = X.shape[0]
@@ -149,7 +150,7 @@ def test_mass_action_properties_65536(gpu, oracle_lib):
     net = networks.mass_action(100, 200, seed=1)
     ts = networks.sample_times(200, 50.0)
     m = gpu.Model.from_arrays(net["left"], net["right"], net["x0"], net["c"], ts)
-    with gpu.Sim(m, 256, 256, seed=0xC0FFEE) as s:
+    with gpu.Sim(m, 256, 256, seed=0xC0FFEE, lanes_per_sim=1) as s:
         s.run()
         g = s.samples()
         gs = s.sim_stats()
@@ -222,7 +223,7 @@ def test_enzyme_cascade_bit_exact(gpu, oracle_lib, force):
     net = networks.enzyme_cascade(20, seed=4)
     ts = networks.sample_times(100, 100.0)
     m = gpu.Model.from_arrays(net["left"], net["right"], net["x0"], net["c"], ts)
-    with gpu.Sim(m, 128, 1, seed=31, force_ssa=force) as s:
+    with gpu.Sim(m, 128, 1, seed=31, force_ssa=force, lanes_per_sim=1) as s:
         s.run()
         out, st, xf = oracle_lib.run(net["left"], net["right"], net["x0"], net["c"], ts, n=128, seed=31,
                                      force_ssa=force, final=True)
@@ -237,7 +238,7 @@ def test_moments_and_binary_output(gpu, tmp_path):
     binary dump round-trips the [sim][sample][species] storage."""
     print("case: moments + binary output", flush=True)
     m = gpu.Model.from_arrays(SCH["left"], SCH["right"], SCH["x0"], SCH["c"], SCH_TS, feed=SCH["feed"])
-    with gpu.Sim(m, 64, 8, seed=9) as s:
+    with gpu.Sim(m, 64, 8, seed=9, lanes_per_sim=1) as s:
         s.run()
         g = s.samples().astype(np.float64)
         mean, sd = s.moments(32)
@@ -295,7 +296,7 @@ def test_replica_group_rows_bit_exact(gpu, oracle_lib):
     x0 = (net["x0"][None, :] * rng.uniform(0.5, 1.5, (16, 10))).astype(np.uint32)
     ts = np.linspace(0.1, 2.0, 20)
     m = gpu.Model.from_arrays(net["left"], net["right"], x0, c, ts)
-    with gpu.Sim(m, 32, 16, seed=12) as s:
+    with gpu.Sim(m, 32, 16, seed=12, lanes_per_sim=1) as s:
         s.run()
         out, st = oracle_lib.run(net["left"], net["right"], x0, c, ts, n=512, seed=12)
         assert np.array_equal(s.samples(), out)
