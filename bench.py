@@ -65,6 +65,7 @@ def main():
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--iters", type=int, default=0)
     ap.add_argument("--ssa-chunk", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per simulation (0: library's choice)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (simulations are independent; "
@@ -89,7 +90,7 @@ def main():
     blocks = (args.sims + threads - 1) // threads
     n = threads * blocks
     sim = ctl.Sim(model, threads, blocks, seed=0xC0FFEE, block_size=args.block_size, slots=args.slots,
-                  iters_per_launch=args.iters, ssa_chunk=args.ssa_chunk)
+                  iters_per_launch=args.iters, ssa_chunk=args.ssa_chunk, lanes_per_sim=args.lanes)
     for w in range(args.warmup):
         sim.run()
         print(f"# warmup {w} done", file=sys.stderr, flush=True)
@@ -140,7 +141,8 @@ def main():
                              "peak_GBps": HBM_PEAK / 1e9,
                              "frac": hbm_bytes / (ker_ms / 1000.0 / args.steps) / HBM_PEAK}},
         "counters": {k: last[k] for k in ("firings", "leaps", "rejections", "switches", "ssa_steps",
-                                          "poisson", "gaps", "absorbed", "lds_state", "slots", "block_size")},
+                                          "poisson", "gaps", "absorbed", "lds_state", "slots", "block_size",
+                                          "lanes_per_sim")},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(net, ts, args.cpu_sims)

@@ -17,8 +17,9 @@
 //  * propensities and criticality per reaction as in the one-lane kernel;
 //  * a0 and every cumulative selection use the lane partial sums (each
 //    lane's reactions in ascending order) combined by a Hillis-Steele
-//    inclusive scan over the G lanes; a selection takes the first lane whose
-//    inclusive sum reaches the threshold, then walks that lane's reactions;
+//    inclusive scan over the G lanes; a selection takes the first lane with
+//    a positive partial whose inclusive sum reaches the threshold (else the
+//    last lane with a positive partial), then walks that lane's reactions;
 //  * tau' per species as in the one-lane kernel, minimum across lanes;
 //  * group-wide draws (tau'', critical pick, SSA uniforms) come from the
 //    event's Philox stream with counter word 3 = 0, lane l's Poisson counts
@@ -131,15 +132,22 @@ __device__ __forceinline__ float gprop(const KParams& p, const int32_t* X, const
     return a;
 }
 
-// First lane whose inclusive sum P reaches w, then the walk through that
-// lane's reactions (which = 0: all; which = 2: critical ones plus reaction 0,
-// the reference's pick walk that starts from a_0, T.cu:1993-2006).  Returns
-// the chosen reaction (group-uniform) or -1.
+// Selection: among the lanes with a positive own partial (Pown), the first
+// whose inclusive sum P reaches w, else the last such lane; then the walk
+// through that lane's reactions from the previous lane's inclusive sum
+// (which = 0: all reactions; which = 2: the critical ones plus reaction 0,
+// the reference's pick walk that starts from a_0, T.cu:1993-2006).  Lanes
+// with a zero partial are skipped: the scan associates differently per lane,
+// so such a lane's inclusive sum can round above its predecessor's.  Only a
+// reaction with a positive propensity is chosen.  Returns the chosen
+// reaction (group-uniform) or -1 when no lane has a positive partial.
 template <int G>
-__device__ __forceinline__ int gselect(const KParams& p, const uint32_t* A, float P, float w, int l, int which) {
-    const uint64_t b = gballot<G>(P >= w);
-    if (b == 0) return -1;
-    const int sel = __builtin_ctzll(b);
+__device__ __forceinline__ int gselect(const KParams& p, const uint32_t* A, float Pown, float P, float w, int l,
+                                       int which) {
+    const uint64_t cand = gballot<G>(Pown > 0.0f);
+    if (cand == 0) return -1;
+    const uint64_t hit = gballot<G>(Pown > 0.0f && P >= w);
+    const int sel = hit ? __builtin_ctzll(hit) : 63 - __builtin_clzll(cand);
     const float excl = __shfl_up(P, 1, G);
     int pick = -1;
     if (l == sel) {
@@ -151,7 +159,7 @@ __device__ __forceinline__ int gselect(const KParams& p, const uint32_t* A, floa
             const float a = aval(u);
             acc = acc + a;
             if (a > 0.0f) last = j;
-            if (w <= acc) { pick = j; break; }
+            if (w <= acc && a > 0.0f) { pick = j; break; }
         }
         if (pick < 0) pick = last;
     }
@@ -235,8 +243,9 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
             const uint32_t u = V.A[j];
             if ((u & 0x80000000u) || j == 0) Q = Q + aval(u);
         }
+        const float Qown = Q;
         Q = gscan<G>(Q, l);
-        pick = gselect<G>(p, V.A, Q, w, l, 2);
+        pick = gselect<G>(p, V.A, Qown, Q, w, l, 2);
     }
     uint64_t fired = 0;
     for (int attempt = 0;; ++attempt) {                                        // T.cu:2011-2050
@@ -331,11 +340,12 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
     for (int step = 0; step < p.ssa_steps; ++step) {
         float P = 0.0f;
         for (int j = l; j < p.R; j += G) P = P + aval(V.A[j]);
+        const float Pown = P;
         P = gscan<G>(P, l);
         const float a0 = __shfl(P, G - 1, G);
         if (a0 == 0.0f) { gabsorb<G>(p, L, X, l); return true; }
         const float r1 = rng_oc(L.gg) * a0;
-        const int j = gselect<G>(p, V.A, P, r1, l, 0);
+        const int j = gselect<G>(p, V.A, Pown, P, r1, l, 0);
         if (j < 0) { gabsorb<G>(p, L, X, l); return true; }    // unreachable for finite a0
         const float dt = exp_rv(L.gg) / a0;
         tacc = tacc + (double)dt;

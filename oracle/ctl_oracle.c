@@ -794,6 +794,7 @@ typedef struct {
     orc_rng* lane;       /* G lane streams, tag 1 + l */
     int32_t* K;          /* R drawn counts */
     float *P, *Q, *T;    /* G lane partials and scan scratch */
+    float* O;            /* G own partials (before the scan) */
 } grpctx;
 
 /* Hillis-Steele inclusive scan over G lanes: v[l] += v[l - d], d = 1, 2, 4... */
@@ -819,22 +820,30 @@ static void g_partials(grpctx* q, float* P, int which) {
     }
 }
 
-/* First lane whose inclusive sum reaches w, then the walk inside it. */
-static int g_select(grpctx* q, const float* P, float w, int which) {
+/* Selection: among the lanes with a positive own partial (Pown), the first
+ * whose inclusive sum (Pincl) reaches w, else the last such lane; then the
+ * walk through that lane's reactions from the previous lane's inclusive sum.
+ * (Lanes with a zero partial are skipped: the scan associates differently
+ * per lane, so such a lane's inclusive sum can round above its predecessor's.) */
+static int g_select(grpctx* q, const float* Pown, const float* Pincl, float w, int which) {
     const simctx* s = q->s;
     const int G = q->G, R = s->m->R;
-    int L = -1;
-    for (int l = 0; l < G; ++l)
-        if (P[l] >= w) { L = l; break; }
+    int L = -1, lastc = -1;
+    for (int l = 0; l < G; ++l) {
+        if (!(Pown[l] > 0.0f)) continue;
+        lastc = l;
+        if (Pincl[l] >= w) { L = l; break; }
+    }
+    if (L < 0) L = lastc;
     if (L < 0) return -1;
-    float acc = L > 0 ? P[L - 1] : 0.0f;
+    float acc = L > 0 ? Pincl[L - 1] : 0.0f;
     int last = -1;
     for (int j = L; j < R; j += G) {
         if (which == 2 && !(s->C[j] || j == 0)) continue;
         const float a = s->Af[j];
         acc = acc + a;
         if (a > 0.0f) last = j;
-        if (w <= acc) return j;
+        if (w <= acc && a > 0.0f) return j;
     }
     return last;
 }
@@ -883,8 +892,9 @@ static int g_leap(grpctx* q) {
     if (tau1 > tau2) {
         const float w = u01_oc(&s->g) * a0c;
         g_partials(q, q->Q, 2);
+        memcpy(q->O, q->Q, sizeof(float) * (size_t)G);
         g_scan(q->Q, q->T, G);
-        pick = g_select(q, q->Q, w, 2);
+        pick = g_select(q, q->O, q->Q, w, 2);
     }
     uint64_t fired;
     for (int attempt = 0;; ++attempt) {
@@ -946,11 +956,13 @@ static void g_ssa(grpctx* q) {
     for (int step = 0; step < J->ssa_steps; ++step) {
         for (int j = 0; j < R; ++j) s->Af[j] = prop_one_f32(s, j, &s->C[j]);
         g_partials(q, q->P, 0);
+        memcpy(q->O, q->P, sizeof(float) * (size_t)G);
         g_scan(q->P, q->T, G);
         const float a0 = q->P[G - 1];
         if (a0 == 0.0f) { absorb(s); return; }
         const float r1 = u01_oc(&s->g) * a0;
-        int j = g_select(q, q->P, r1, 0);
+        int j = g_select(q, q->O, q->P, r1, 0);
+        if (j < 0) { absorb(s); return; }         /* unreachable for a0 > 0 */
         const float dt = exp_rv(&s->g) / a0;
         tacc = tacc + (double)dt;
         const double tn = s->t + tacc;
@@ -1021,7 +1033,7 @@ int orc_run(const orc_net* net, const orc_job* J, uint32_t* storage, orc_stat* s
         const int G = J->group > 0 ? J->group : 1;
         orc_rng* lanes = (orc_rng*)calloc((size_t)G, sizeof(orc_rng));
         int32_t* Kc = (int32_t*)malloc(sizeof(int32_t) * R);
-        float* gP = (float*)malloc(sizeof(float) * 3 * (size_t)G);
+        float* gP = (float*)malloc(sizeof(float) * 4 * (size_t)G);
 #pragma omp for schedule(dynamic, 16)
         for (int i = 0; i < J->n; ++i) {
             simctx s;
@@ -1040,7 +1052,7 @@ int orc_run(const orc_net* net, const orc_job* J, uint32_t* storage, orc_stat* s
             s.g.sim = J->sim_base + (uint32_t)i;
             s.g.pos = 4;
             if (J->group > 0) {
-                grpctx q = {&s, G, lanes, Kc, gP, gP + G, gP + 2 * G};
+                grpctx q = {&s, G, lanes, Kc, gP, gP + G, gP + 2 * G, gP + 3 * G};
                 for (int l = 0; l < G; ++l) {
                     lanes[l] = s.g;
                     lanes[l].tag = 1u + (uint32_t)l;

@@ -21,7 +21,8 @@
  * belong to lane j mod G and s mod G.  Same per-reaction propensities and
  * per-species tau candidates as REPLAY; a0 and the selection sums are the
  * lane partial sums (ascending reactions) combined by a Hillis-Steele
- * inclusive scan over the G lanes; each lane draws its own reactions'
+ * inclusive scan over the G lanes (a selection picks among the lanes with a
+ * positive partial); each lane draws its own reactions'
  * counts from its own Philox stream (counter word 3 = 1 + lane); the
  * update and its negativity test run per species over the species-major
  * fear entries (the same rejections as the reaction-major order).

@@ -56,12 +56,13 @@ def _chi2_p(counts, p):
     return stats.chi2.sf(((counts - exp) ** 2 / exp).sum(), len(p) - 1)
 
 
-@pytest.mark.parametrize("G", [4, 8, 16])
+@pytest.mark.parametrize("G", [4, 8, 16, 64])
 def test_group_ssa_selection_is_multinomial(oracle_lib, G):
     """Exact SSA through the group selection: every X molecule ends in
     channel j with probability c_j / sum c.  37 channels over G lanes put
-    several reactions in every lane, so a wrong lane prefix, scan order or
-    in-lane walk biases the channel counts."""
+    several reactions in every lane (G <= 16), so a wrong lane prefix, scan
+    order or in-lane walk biases the channel counts; G = 64 leaves 27 lanes
+    without reactions, whose zero partials the selection must skip."""
     M = 37
     left, right, c = _channels(M, 11)
     x0 = np.zeros(M + 1, np.uint32)
@@ -74,7 +75,7 @@ def test_group_ssa_selection_is_multinomial(oracle_lib, G):
     assert _chi2_p(counts, c / c.sum()) > 1e-3
 
 
-@pytest.mark.parametrize("G", [4, 8])
+@pytest.mark.parametrize("G", [4, 8, 64])
 def test_group_critical_pick_is_multinomial(oracle_lib, G):
     """X0 = 9 < nc: every channel is critical, tau' is infinite and each leap
     fires exactly one reaction chosen by the critical pick, an exact SSA.
