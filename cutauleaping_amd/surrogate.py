@@ -177,3 +177,71 @@ def screened_sweep(problem: SweepProblem, surrogate: Surrogate, u, *, replicas: 
     t2 = time.perf_counter()
     return dict(mean=mean, sd=sd, fallback=fb, spread=spread, seconds=t2 - t0, surrogate_seconds=t1 - t0,
                 simulator_seconds=sim_s)
+
+
+def sweep_study(problem: SweepProblem, *, n_train=8192, n_test=512, replicas=256, judge_replicas=None,
+                members=5, epochs=150, batch=256, lr=1e-3, threshold=0.05, seed=1, gpu=0, log=None,
+                sim_opts=None) -> dict:
+    """Data generation, ensemble training and a judged, timed sweep.
+
+    Training: n_train parameter sets x `replicas` simulations, reduced to
+    mean/sd on the device.  Held out: n_test new sets, simulated with
+    `judge_replicas` (default `replicas`) to judge, and timed as the plain
+    sweep.  The screened sweep predicts every held-out set, simulates those
+    whose ensemble spread exceeds `threshold`, and is judged on the same
+    values.  Returns timings, errors (relative L2 per set, of mean and of sd)
+    and the error of the trivial predictor (the training mean) for scale."""
+    sim_opts = dict(sim_opts or {})
+    judge_replicas = judge_replicas or replicas
+    say = log or (lambda *a: None)
+    rng = np.random.default_rng(seed)
+    u_tr = problem.sample_u(n_train, rng)
+    u_te = problem.sample_u(n_test, rng)
+    t0 = time.perf_counter()
+    chunk = max(1, (1 << 20) // replicas)           # sets per launch: about 2^20 simulations
+    means, sds, gen_fir = [], [], 0
+    for i in range(0, n_train, chunk):
+        m, s, st, _ = problem.simulate(u_tr[i:i + chunk], replicas, offset=i * replicas, seed=seed, gpu=gpu,
+                                       **sim_opts)
+        means.append(m)
+        sds.append(s)
+        gen_fir += st["firings"]
+        say(f"# training data {min(i + chunk, n_train)}/{n_train} sets")
+    gen_s = time.perf_counter() - t0
+    y_tr = problem.features(np.concatenate(means), np.concatenate(sds))
+    sur = Surrogate(WIDTHS if problem.n_in == 32 and problem.n_out == 1000 else
+                    (problem.n_in, 256, 256, 256, problem.n_out), members=members, gpu=gpu, seed=seed)
+    t0 = time.perf_counter()
+    losses = sur.fit(u_tr, y_tr, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    train_s = time.perf_counter() - t0
+    say(f"# trained {members} members in {train_s:.1f}s, final losses {losses[:, -1].round(4).tolist()}")
+    # plain sweep (timed) on the held-out sets; its values are the judge
+    jm, js, jst, plain_s = problem.simulate(u_te, judge_replicas, offset=(1 << 30), seed=seed + 1, gpu=gpu,
+                                            **sim_opts)
+    if judge_replicas != replicas:
+        _, _, _, plain_s = problem.simulate(u_te, replicas, offset=(1 << 29), seed=seed + 2, gpu=gpu,
+                                            **sim_opts)
+    say(f"# plain sweep of {n_test} sets: {plain_s:.2f}s")
+    res = screened_sweep(problem, sur, u_te, replicas=replicas, threshold=threshold, offset=(1 << 28),
+                         seed=seed + 3, gpu=gpu, **sim_opts)
+    e_mean = relative_error(res["mean"], jm)
+    e_sd = relative_error(res["sd"], js)
+    # the surrogate alone (no fallback) and the trivial predictor
+    y_hat, _ = sur.predict(u_te)
+    sm, ss = problem.unfeatures(y_hat)
+    mm, ms = problem.unfeatures(np.repeat(y_tr.mean(axis=0, keepdims=True), n_test, axis=0))
+    sur.close()
+    return dict(
+        n_train=n_train, n_test=n_test, replicas=replicas, judge_replicas=judge_replicas, members=members,
+        generation_s=gen_s, generation_firings=int(gen_fir), train_s=train_s,
+        final_loss=float(losses[:, -1].mean()),
+        plain_sweep_s=plain_s, screened_sweep_s=res["seconds"], surrogate_s=res["surrogate_seconds"],
+        fallback_s=res["simulator_seconds"], fallback_fraction=float(res["fallback"].mean()),
+        speedup=plain_s / res["seconds"],
+        err_mean=float(e_mean.mean()), err_sd=float(e_sd.mean()),
+        err_mean_p95=float(np.percentile(e_mean, 95)), err_sd_p95=float(np.percentile(e_sd, 95)),
+        surrogate_only_err_mean=float(relative_error(sm, jm).mean()),
+        surrogate_only_err_sd=float(relative_error(ss, js).mean()),
+        trivial_err_mean=float(relative_error(mm, jm).mean()),
+        trivial_err_sd=float(relative_error(ms, js).mean()),
+    )
