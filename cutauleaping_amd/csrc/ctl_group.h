@@ -7,7 +7,7 @@
 //     X   int32 [2][S]   state and trial state (parity bit swaps them)
 //     A   uint32[R]      propensity (FP32 bits) | critical flag << 31
 //     Kc  int32 [R]      counts drawn for the current leap attempt
-//     Cs  float [R]      rate constants of the simulation
+//     Q   int32 [R]      reactions queued for a Poisson draw (lambda >= 1)
 // so every table-driven access is an LDS access instead of a dependent HBM
 // load.  Group-wide values (a0, tau, the chosen reaction) are computed with
 // cross-lane shuffles and are identical in every lane of the group, so the
@@ -22,8 +22,11 @@
 //    last lane with a positive partial), then walks that lane's reactions;
 //  * tau' per species as in the one-lane kernel, minimum across lanes;
 //  * group-wide draws (tau'', critical pick, SSA uniforms) come from the
-//    event's Philox stream with counter word 3 = 0, lane l's Poisson counts
-//    from the stream with word 3 = 1 + l;
+//    event's Philox stream with counter word 3 = 0; lane l's small-lambda
+//    Poisson process from the stream with word 3 = 1 + l (first gap drawn at
+//    the start of the attempt); the count of a reaction with lambda >= 1
+//    from the reaction's own stream (word 3 = 2^31 | j), so those draws can
+//    be queued and spread over the group's lanes;
 //  * the leap update and its negativity / int32 test run per species over
 //    the species-major fear entries in ascending reaction order, which
 //    rejects exactly when the reaction-major order of test_update
@@ -51,24 +54,84 @@ __device__ __forceinline__ uint64_t gballot(bool pred) {
     return (b >> base) & ((1ull << G) - 1ull);
 }
 
-// Hillis-Steele inclusive scan over the group: v_l <- v_{l-d} + v_l.
+// Cross-lane moves inside a row of 16 lanes through DPP (a VALU operand
+// modifier, no LDS round trip).  Lanes whose source is outside the row read 0.
+template <int CTRL>
+__device__ __forceinline__ float dppf(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+enum : int {
+    DPP_QP_1133 = 0xF5,       // quad_perm [1,1,3,3]
+    DPP_QP_3333 = 0xFF,       // quad_perm [3,3,3,3]
+    DPP_QP_1032 = 0xB1,       // quad_perm [1,0,3,2]  (xor 1)
+    DPP_QP_2301 = 0x4E,       // quad_perm [2,3,0,1]  (xor 2)
+    DPP_ROW_SHL4 = 0x104,
+    DPP_ROW_SHR1 = 0x111,
+    DPP_ROW_SHR2 = 0x112,
+    DPP_ROW_SHR4 = 0x114,
+    DPP_ROW_SHR8 = 0x118,
+    DPP_ROW_MIRROR = 0x140,
+    DPP_ROW_HALF_MIRROR = 0x141,
+};
+
+// Hillis-Steele inclusive scan over the group: v_l <- v_{l-d} + v_l for
+// d = 1, 2, 4, ...  A group of at most 16 lanes lies inside one DPP row, so
+// every step is a row shift; wider groups span rows and use shuffles.
 template <int G>
 __device__ __forceinline__ float gscan(float v, int l) {
+    float t;
+    if (G <= 16) {
+        if (G >= 2) { t = dppf<DPP_ROW_SHR1>(v); if (l >= 1) v = t + v; }
+        if (G >= 4) { t = dppf<DPP_ROW_SHR2>(v); if (l >= 2) v = t + v; }
+        if (G >= 8) { t = dppf<DPP_ROW_SHR4>(v); if (l >= 4) v = t + v; }
+        if (G >= 16) { t = dppf<DPP_ROW_SHR8>(v); if (l >= 8) v = t + v; }
+    } else {
 #pragma unroll
-    for (int d = 1; d < G; d <<= 1) {
-        const float t = __shfl_up(v, d, G);
-        if (l >= d) v = t + v;
+        for (int d = 1; d < G; d <<= 1) {
+            t = __shfl_up(v, d, G);
+            if (l >= d) v = t + v;
+        }
     }
     return v;
 }
 
+__device__ __forceinline__ float rdlane(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// The value of the group's last lane, in every lane of the group.
+template <int G>
+__device__ __forceinline__ float gbcast_last(float v, int l) {
+    if (G == 2) return dppf<DPP_QP_1133>(v);
+    if (G == 4) return dppf<DPP_QP_3333>(v);
+    if (G == 8) {
+        const float t = dppf<DPP_QP_3333>(v);
+        const float u = dppf<DPP_ROW_SHL4>(t);
+        return l < 4 ? u : t;
+    }
+    const int wl = (int)(threadIdx.x & 63u);
+    if (G == 16) {
+        const float r0 = rdlane(v, 15), r1 = rdlane(v, 31), r2 = rdlane(v, 47), r3 = rdlane(v, 63);
+        const int q = wl >> 4;
+        return q == 0 ? r0 : (q == 1 ? r1 : (q == 2 ? r2 : r3));
+    }
+    if (G == 32) {
+        const float r0 = rdlane(v, 31), r1 = rdlane(v, 63);
+        return wl < 32 ? r0 : r1;
+    }
+    return rdlane(v, 63);
+}
+
+// Minimum over the group (exact in any order).
 template <int G>
 __device__ __forceinline__ float gmin(float v) {
-#pragma unroll
-    for (int m = G / 2; m >= 1; m >>= 1) {
-        const float o = __shfl_xor(v, m, G);
-        v = o < v ? o : v;
-    }
+    float o;
+    if (G >= 2) { o = dppf<DPP_QP_1032>(v); v = o < v ? o : v; }
+    if (G >= 4) { o = dppf<DPP_QP_2301>(v); v = o < v ? o : v; }
+    if (G >= 8) { o = dppf<DPP_ROW_HALF_MIRROR>(v); v = o < v ? o : v; }
+    if (G >= 16) { o = dppf<DPP_ROW_MIRROR>(v); v = o < v ? o : v; }
+    if (G >= 32) { o = __shfl_xor(v, 16, G); v = o < v ? o : v; }
+    if (G >= 64) { o = __shfl_xor(v, 32, G); v = o < v ? o : v; }
     return v;
 }
 
@@ -93,6 +156,7 @@ struct GLane {
     uint64_t firings, ssa_steps;
     uint32_t leaps, rejections, switches, absorbed;
     uint64_t poisson, gaps;
+    const float* crate;      // the simulation's rate constants (HBM, cached)
     Rng gg;                  // group stream (word 3 = 0)
     Rng lg;                  // this lane's stream (word 3 = 1 + lane)
 };
@@ -101,13 +165,13 @@ struct GView {
     int32_t* Xb;             // [2][S]
     uint32_t* A;             // [R]
     int32_t* Kc;             // [R]
-    float* Cs;               // [R]
+    int32_t* Q;              // [R]
 };
 
 // Propensity of reaction j from the group's LDS state; the arithmetic of
 // prop_one (CalculatePropensity, T.cu:1443-1470, in FP32).
-__device__ __forceinline__ float gprop(const KParams& p, const int32_t* X, const float* Cs, int j, bool& crit) {
-    float a = Cs[j];
+__device__ __forceinline__ float gprop(const KParams& p, const int32_t* X, const float* crate, int j, bool& crit) {
+    float a = crate[j];
     bool c = false;
     const uint32_t nc = (uint32_t)p.nc;
     const uint32_t h1 = p.Lptr[j + 1];
@@ -125,8 +189,8 @@ __device__ __forceinline__ float gprop(const KParams& p, const int32_t* X, const
         } else {
             for (int q = 1; q <= v; ++q) a = a * ((float)(x - q + 1) / (float)q);
         }
-        const uint32_t fl = (uint32_t)x / (uint32_t)v;
-        c = c || (fl > 0u && fl < nc);
+        // 0 < floor(x / v) < nc, as compares: v <= x < nc * v
+        c = c || ((uint32_t)x >= (uint32_t)v && (uint64_t)(uint32_t)x < (uint64_t)nc * (uint32_t)v);
     }
     crit = c;
     return a;
@@ -148,7 +212,7 @@ __device__ __forceinline__ int gselect(const KParams& p, const uint32_t* A, floa
     if (cand == 0) return -1;
     const uint64_t hit = gballot<G>(Pown > 0.0f && P >= w);
     const int sel = hit ? __builtin_ctzll(hit) : 63 - __builtin_clzll(cand);
-    const float excl = __shfl_up(P, 1, G);
+    const float excl = G <= 16 ? dppf<DPP_ROW_SHR1>(P) : __shfl_up(P, 1, G);
     int pick = -1;
     if (l == sel) {
         float acc = l > 0 ? excl : 0.0f;
@@ -191,7 +255,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
     float P = 0.0f, PC = 0.0f;
     for (int j = l; j < p.R; j += G) {
         bool crit;
-        const float a = gprop(p, X, V.Cs, j, crit);
+        const float a = gprop(p, X, L.crate, j, crit);
         V.A[j] = __float_as_uint(a) | (crit ? 0x80000000u : 0u);
         P = P + a;
         if (crit) PC = PC + a;
@@ -199,8 +263,8 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
     gsync();
     P = gscan<G>(P, l);
     PC = gscan<G>(PC, l);
-    const float a0 = __shfl(P, G - 1, G);
-    const float a0c = __shfl(PC, G - 1, G);
+    const float a0 = gbcast_last<G>(P, l);
+    const float a0c = gbcast_last<G>(PC, l);
     if (a0 == 0.0f) { gabsorb<G>(p, L, X, l); return LEAP_ABSORB; }
     // tau' (T.cu:1860-1922): the one-lane kernel's per-species values, min over lanes
     float tau1 = 3.40282347e+38f;
@@ -250,30 +314,55 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
     uint64_t fired = 0;
     for (int attempt = 0;; ++attempt) {                                        // T.cu:2011-2050
         if (attempt > 0) { e = L.evc++; L.rejections++; }
+        // counts: critical reactions (the pick), the small-lambda process of
+        // this lane (its first gap drawn up front), and a queue of the
+        // reactions with lambda >= 1, drawn afterwards by the group's lanes in
+        // parallel, each from the reaction's own stream (word 3 = 2^31 | j)
         rng_event(L.lg, e);
         uint64_t f = 0;
-        double cum = 0.0, pos = -1.0;
-        for (int j = l; j < p.R; j += G) {
-            const uint32_t u = V.A[j];
-            int32_t k;
-            if (u & 0x80000000u) {
-                k = (attempt == 0 && j == pick) ? 1 : 0;
-            } else {
-                const float lam = tau * __uint_as_float(u);
-                if (lam < 1.0f) {
-                    k = 0;
-                    if (lam > 0.0f) {
-                        if (pos < 0.0) { pos = (double)exp_rv(L.lg); L.gaps++; }
-                        cum = cum + (double)lam;
-                        while (pos < cum) { ++k; pos = pos + (double)exp_rv(L.lg); L.gaps++; }
-                    }
+        double cum = 0.0, pos = (double)exp_rv(L.lg);
+        L.gaps++;
+        uint32_t nq = 0;
+        const int nit = (p.R + G - 1) / G;
+        for (int i = 0; i < nit; ++i) {
+            const int j = l + i * G;
+            bool need = false;
+            if (j < p.R) {
+                const uint32_t u = V.A[j];
+                int32_t k = 0;
+                if (u & 0x80000000u) {
+                    k = (attempt == 0 && j == pick) ? 1 : 0;
                 } else {
-                    k = poisson(L.lg, lam, recip);
-                    L.poisson++;
+                    const float lam = tau * __uint_as_float(u);
+                    if (lam < 1.0f) {
+                        if (lam > 0.0f) {
+                            cum = cum + (double)lam;
+                            while (pos < cum) { ++k; pos = pos + (double)exp_rv(L.lg); L.gaps++; }
+                        }
+                    } else {
+                        need = true;
+                    }
+                }
+                if (!need) {
+                    V.Kc[j] = k;
+                    f += (uint64_t)k;
                 }
             }
+            const uint64_t m = gballot<G>(need);
+            if (need) V.Q[nq + (uint32_t)__popcll(m & ((1ull << l) - 1ull))] = j;
+            nq += (uint32_t)__popcll(m);
+        }
+        gsync();
+        for (uint32_t qi = (uint32_t)l; qi < nq; qi += G) {
+            const int j = V.Q[qi];
+            const float lam = tau * __uint_as_float(V.A[j]);
+            Rng gr = L.gg;
+            gr.tag = 0x80000000u | (uint32_t)j;
+            rng_event(gr, e);
+            const int32_t k = poisson(gr, lam, recip);
             V.Kc[j] = k;
             f += (uint64_t)k;
+            L.poisson++;
         }
         gsync();
         bool bad = false;
@@ -333,7 +422,7 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
     double tacc = 0.0;
     for (int j = l; j < p.R; j += G) {
         bool crit;
-        const float a = gprop(p, X, V.Cs, j, crit);
+        const float a = gprop(p, X, L.crate, j, crit);
         V.A[j] = __float_as_uint(a) | (crit ? 0x80000000u : 0u);
     }
     gsync();
@@ -342,7 +431,7 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
         for (int j = l; j < p.R; j += G) P = P + aval(V.A[j]);
         const float Pown = P;
         P = gscan<G>(P, l);
-        const float a0 = __shfl(P, G - 1, G);
+        const float a0 = gbcast_last<G>(P, l);
         if (a0 == 0.0f) { gabsorb<G>(p, L, X, l); return true; }
         const float r1 = rng_oc(L.gg) * a0;
         const int j = gselect<G>(p, V.A, Pown, P, r1, l, 0);
@@ -374,7 +463,7 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
         for (uint32_t d = p.Dptr[j] + (uint32_t)l; d < d1; d += G) {
             const int r = (int)p.Dent[d];
             bool crit;
-            const float a = gprop(p, X, V.Cs, r, crit);
+            const float a = gprop(p, X, L.crate, r, crit);
             V.A[r] = __float_as_uint(a) | (crit ? 0x80000000u : 0u);
         }
         gsync();
@@ -383,10 +472,8 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
     return false;
 }
 
-template <int G>
-__device__ __forceinline__ void gload_rates(const KParams& p, const GView& V, int sim, int l) {
-    for (int j = l; j < p.R; j += G)
-        V.Cs[j] = p.c_bcast ? p.c_bcast[j] : p.c_sims[(size_t)(sim / p.c_div) * p.R + j];
+__device__ __forceinline__ const float* grates(const KParams& p, int sim) {
+    return p.c_bcast ? p.c_bcast : p.c_sims + (size_t)(sim / p.c_div) * p.R;
 }
 
 // Next simulation from the queue (PrepareVariables, T.cu:1269-1297).
@@ -408,7 +495,7 @@ __device__ __forceinline__ bool gfetch(const KParams& p, GLane& L, const GView& 
     L.gg.pos = L.lg.pos = 4;
     for (int s = l; s < p.S; s += G)
         V.Xb[s] = (int32_t)(p.x0_bcast ? p.x0_bcast[s] : p.x0_sims[(size_t)(q / (uint32_t)p.x0_div) * p.S + s]);
-    gload_rates<G>(p, V, (int)q, l);
+    L.crate = grates(p, (int)q);
     gsync();
     return true;
 }
@@ -459,7 +546,7 @@ __global__ __launch_bounds__(256) void group_kernel(KParams p) {
     V.Xb = (int32_t*)(smem + (size_t)gib * words);
     V.A = (uint32_t*)(V.Xb + 2 * p.S);
     V.Kc = (int32_t*)(V.A + p.R);
-    V.Cs = (float*)(V.Kc + p.R);
+    V.Q = V.Kc + p.R;
     L.gg.k0 = L.lg.k0 = p.key0;
     L.gg.k1 = L.lg.k1 = p.key1;
     L.gg.tag = 0u;
@@ -483,7 +570,7 @@ __global__ __launch_bounds__(256) void group_kernel(KParams p) {
         L.gg.sim = L.lg.sim = p.sim_base + (uint32_t)L.sim;
         L.gg.pos = L.lg.pos = 4;
         for (int s = l; s < p.S; s += G) V.Xb[s] = p.X[(size_t)s0 * p.S + s];
-        gload_rates<G>(p, V, L.sim, l);
+        L.crate = grates(p, L.sim);
         gsync();
     }
     for (int it = 0; it < p.iters; ++it) {

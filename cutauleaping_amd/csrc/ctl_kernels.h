@@ -179,8 +179,8 @@ __device__ __forceinline__ uint32_t prop_one(const KParams& p, const View& V, co
         } else {
             for (int q = 1; q <= v; ++q) a = a * ((float)(x - q + 1) / (float)q);
         }
-        const uint32_t fl = (uint32_t)x / (uint32_t)v;
-        crit = crit || (fl > 0u && fl < nc);
+        // 0 < floor(x / v) < nc, as compares: v <= x < nc * v
+        crit = crit || ((uint32_t)x >= (uint32_t)v && (uint64_t)(uint32_t)x < (uint64_t)nc * (uint32_t)v);
     }
     *aout = a;
     return __float_as_uint(a) | (crit ? 0x80000000u : 0u);

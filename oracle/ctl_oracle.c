@@ -900,10 +900,14 @@ static int g_leap(grpctx* q) {
     for (int attempt = 0;; ++attempt) {
         if (attempt > 0) { e = s->evc++; s->st.rejections++; }
         fired = 0;
+        /* per lane: critical reactions (the pick) and the lane's small-lambda
+         * process, whose first gap is drawn up front; reactions with
+         * lambda >= 1 draw from their own stream (word 3 = 2^31 | j) */
         for (int l = 0; l < G; ++l) {
             orc_rng* g = &q->lane[l];
             rng_begin(g, e);
-            double cum = 0.0, pos = -1.0;
+            double cum = 0.0, pos = (double)exp_rv(g);
+            s->st.gaps++;
             for (int j = l; j < R; j += G) {
                 int32_t k;
                 if (s->C[j]) {
@@ -913,12 +917,14 @@ static int g_leap(grpctx* q) {
                     if (lam < 1.0f) {
                         k = 0;
                         if (lam > 0.0f) {
-                            if (pos < 0.0) { pos = (double)exp_rv(g); s->st.gaps++; }
                             cum = cum + (double)lam;
                             while (pos < cum) { ++k; pos = pos + (double)exp_rv(g); s->st.gaps++; }
                         }
                     } else {
-                        k = poisson(g, lam);
+                        orc_rng rj = s->g;
+                        rj.tag = 0x80000000u | (uint32_t)j;
+                        rng_begin(&rj, e);
+                        k = poisson(&rj, lam);
                         s->st.poisson++;
                     }
                 }

@@ -22,9 +22,10 @@
  * per-species tau candidates as REPLAY; a0 and the selection sums are the
  * lane partial sums (ascending reactions) combined by a Hillis-Steele
  * inclusive scan over the G lanes (a selection picks among the lanes with a
- * positive partial); each lane draws its own reactions'
- * counts from its own Philox stream (counter word 3 = 1 + lane); the
- * update and its negativity test run per species over the species-major
+ * positive partial); the small-lambda process of each lane uses the
+ * lane's Philox stream (counter word 3 = 1 + lane, first gap drawn at the
+ * start of the attempt) and every Poisson count (lambda >= 1) the
+ * reaction's own stream (word 3 = 2^31 | j); the update and its negativity test run per species over the species-major
  * fear entries (the same rejections as the reaction-major order).
  *
  * Random numbers: Philox4x32-10 (Salmon et al., SC'11) keyed by the seed,
