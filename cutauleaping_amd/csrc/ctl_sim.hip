@@ -515,8 +515,8 @@ int ctl_sim_run(ctl_sim* s) {
         HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1), "hipEventElapsedTime");
         total_ms += ms;
         S.launches++;
-        if (s->opts.verbose)
-            std::printf(" * launch %d: %u/%d simulations done (%.3f ms)\n", S.launches, done, s->n, ms);
+        if (s->opts.verbose)   // progress on stderr (unbuffered): long runs report as they go
+            std::fprintf(stderr, " * launch %d: %u/%d simulations done (%.3f ms)\n", S.launches, done, s->n, ms);
         if (S.launches >= max_launches) return ctl::fail(CTL_E_STATE, "simulation did not finish");
     }
     S.kernel_ms = total_ms;

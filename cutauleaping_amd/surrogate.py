@@ -111,6 +111,7 @@ class SweepProblem:
         rows = y.shape[0]
         h = self.n_out // 2
         K, SS = len(self.ts), len(self.observed)
+        y = np.asarray(y, np.float64)
         m = np.expm1(np.maximum(y[:, :h], 0.0)).reshape(rows, K, SS)
         s = np.expm1(np.maximum(y[:, h:], 0.0)).reshape(rows, K, SS)
         return m, s
