@@ -1,7 +1,14 @@
 // ctl_batch.hip -- many independent models of uneven size in one persistent
 // launch (no counterpart in the reference, which runs one model per process).
 //
-// Work is split into chunks of 64 replicas of one model, ordered by an
+// Two kernels.  batch_kernel runs one simulation per lane (small models).
+// gbatch_kernel runs the group algorithm with one 64-lane wavefront per
+// simulation and the simulation's state in LDS (large models): a wavefront
+// takes the next simulation from one global counter over all models in
+// estimated-cost order (largest first), so its model is wave-uniform and its
+// descriptor lives in scalar registers.
+//
+// batch_kernel: work is split into chunks of 64 replicas of one model, ordered by an
 // estimated per-replica cost, largest first.  A wavefront takes the next
 // chunk's model as its current model -- so the model tables stay
 // wave-uniform scalar loads -- and each of its lanes then steals replicas of
@@ -20,6 +27,7 @@
 #include <numeric>
 #include <vector>
 
+#include "ctl_group.h"
 #include "ctl_kernels.h"
 
 #pragma clang fp contract(off)
@@ -46,6 +54,15 @@ __device__ __forceinline__ KParams desc_of(const BatchParams& bp, int m) {
     static_assert(sizeof(KParams) % 4 == 0, "descriptor copied as words");
     KParams p;
     const ctab_u32 src = CT_U32(reinterpret_cast<const uint32_t*>(bp.desc + m));
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&p);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(KParams) / 4); ++i) dst[i] = src[i];
+    return p;
+}
+
+__device__ __forceinline__ KParams desc_of_b(const KParams* desc, int m) {
+    KParams p;
+    const ctab_u32 src = CT_U32(reinterpret_cast<const uint32_t*>(desc + m));
     uint32_t* dst = reinterpret_cast<uint32_t*>(&p);
 #pragma unroll
     for (int i = 0; i < (int)(sizeof(KParams) / 4); ++i) dst[i] = src[i];
@@ -129,6 +146,151 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchParams bp) {
     if (L.sim >= 0) store_lane(desc_of(bp, m), L);
 }
 
+
+// ---------------------------------------------------------------------------
+// Group batch: one simulation per 64-lane wavefront (group algorithm, G = 64).
+struct GBatchParams {
+    const KParams* desc;        // per-model descriptors (device)
+    const int32_t* order;       // models in cost order
+    const int64_t* cum;         // cumulative replicas in that order [nmodels + 1]
+    int nmodels;
+    int64_t total;
+    uint32_t* next;             // global simulation counter
+    int ngroups;                // resident group slots (one wavefront each)
+    int iters;
+    int maxS;
+    // parked state per group slot
+    int32_t* slot_model;
+    int32_t* slot_sim;          // replica index, -1 none, -2 queue exhausted
+    double* t;
+    uint32_t* cursor;
+    uint32_t* evc;
+    uint64_t* firings;
+    uint64_t* ssa_steps;
+    uint32_t* leaps;
+    uint32_t* rejections;
+    uint32_t* switches;
+    uint64_t* poisson;          // [slot][64]
+    uint64_t* gaps;             // [slot][64]
+    int32_t* X;                 // [slot][maxS]
+};
+
+__device__ __forceinline__ GView gview(uint32_t* smem, int S, int R) {
+    GView V;
+    V.Xb = (int32_t*)smem;
+    V.A = (uint32_t*)(V.Xb + 2 * S);
+    V.Kc = (int32_t*)(V.A + R);
+    V.Q = V.Kc + R;
+    return V;
+}
+
+__global__ __launch_bounds__(64) void gbatch_kernel(GBatchParams bp) {
+    constexpr int G = 64;
+    extern __shared__ uint32_t smem[];
+    __shared__ float recip[64];
+    recip[threadIdx.x] = c_recip[threadIdx.x];
+    __syncthreads();
+    const int l = (int)threadIdx.x;
+    const int slot = (int)blockIdx.x;
+    if (slot >= bp.ngroups) return;
+    GLane L;
+    L.slot = slot;
+    L.gg.tag = 0u;
+    L.lg.tag = 1u + (uint32_t)l;
+    int m = __builtin_amdgcn_readfirstlane(bp.slot_model[slot]);
+    L.sim = __builtin_amdgcn_readfirstlane(bp.slot_sim[slot]);
+    if (L.sim == -2) return;
+    KParams p;
+    GView V;
+    if (L.sim >= 0) {                    // resume a parked simulation
+        p = desc_of_b(bp.desc, m);
+        V = gview(smem, p.S, p.R);
+        L.t = bp.t[slot];
+        L.cursor = bp.cursor[slot];
+        L.evc = bp.evc[slot];
+        L.xb = 0;
+        L.firings = bp.firings[slot];
+        L.ssa_steps = bp.ssa_steps[slot];
+        L.leaps = bp.leaps[slot];
+        L.rejections = bp.rejections[slot];
+        L.switches = bp.switches[slot];
+        L.absorbed = 0;
+        L.poisson = bp.poisson[(size_t)slot * G + l];
+        L.gaps = bp.gaps[(size_t)slot * G + l];
+        L.gg.k0 = L.lg.k0 = p.key0;
+        L.gg.k1 = L.lg.k1 = p.key1;
+        L.gg.sim = L.lg.sim = p.sim_base + (uint32_t)L.sim;
+        L.gg.pos = L.lg.pos = 4;
+        for (int s = l; s < p.S; s += G) V.Xb[s] = bp.X[(size_t)slot * bp.maxS + s];
+        L.crate = grates(p, L.sim);
+        gsync();
+    }
+    for (int it = 0; it < bp.iters; ++it) {
+        if (L.sim < 0) {                 // next simulation of any model
+            uint32_t q = 0;
+            if (l == 0) q = atomicAdd(bp.next, 1u);
+            q = __builtin_amdgcn_readfirstlane(q);
+            if ((int64_t)q >= bp.total) { L.sim = -2; break; }
+            int lo = 0, hi = bp.nmodels - 1;           // last k with cum[k] <= q
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if ((int64_t)q >= CT_I64(bp.cum)[mid]) lo = mid; else hi = mid - 1;
+            }
+            lo = __builtin_amdgcn_readfirstlane(lo);
+            m = __builtin_amdgcn_readfirstlane(CT_I32(bp.order)[lo]);
+            const uint32_t r = (uint32_t)((int64_t)q - CT_I64(bp.cum)[lo]);
+            p = desc_of_b(bp.desc, m);
+            V = gview(smem, p.S, p.R);
+            L.sim = (int)r;
+            L.t = 0.0;
+            L.cursor = 0;
+            L.evc = 0;
+            L.xb = 0;
+            L.firings = L.ssa_steps = 0;
+            L.leaps = L.rejections = L.switches = L.absorbed = 0;
+            L.poisson = L.gaps = 0;
+            L.gg.k0 = L.lg.k0 = p.key0;
+            L.gg.k1 = L.lg.k1 = p.key1;
+            L.gg.sim = L.lg.sim = p.sim_base + r;
+            L.gg.pos = L.lg.pos = 4;
+            for (int s = l; s < p.S; s += G)
+                V.Xb[s] = (int32_t)(p.x0_bcast ? p.x0_bcast[s] : p.x0_sims[(size_t)(r / (uint32_t)p.x0_div) * p.S + s]);
+            L.crate = grates(p, (int)r);
+            gsync();
+        }
+        if (p.force_ssa) {
+            if (L.t >= p.tmax || gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);
+            continue;
+        }
+        const int r = gleap<G>(p, L, V, l, recip);
+        if (r == LEAP_DONE || r == LEAP_ABSORB) {
+            gfinish<G>(p, L, V, l);
+        } else if (r == LEAP_SWITCH) {
+            if (gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);
+        }
+    }
+    if (L.sim >= 0) {                    // park
+        const int32_t* X = V.Xb + (L.xb ? p.S : 0);
+        for (int s = l; s < p.S; s += G) bp.X[(size_t)slot * bp.maxS + s] = X[s];
+        bp.poisson[(size_t)slot * G + l] = L.poisson;
+        bp.gaps[(size_t)slot * G + l] = L.gaps;
+        if (l == 0) {
+            bp.t[slot] = L.t;
+            bp.cursor[slot] = L.cursor;
+            bp.evc[slot] = L.evc;
+            bp.firings[slot] = L.firings;
+            bp.ssa_steps[slot] = L.ssa_steps;
+            bp.leaps[slot] = L.leaps;
+            bp.rejections[slot] = L.rejections;
+            bp.switches[slot] = L.switches;
+        }
+    }
+    if (l == 0) {
+        bp.slot_model[slot] = m;
+        bp.slot_sim[slot] = L.sim;
+    }
+}
+
 }  // namespace ctl
 
 // ===========================================================================
@@ -147,6 +309,9 @@ struct ctl_batch {
     std::vector<int> K, SS;
     KParams* d_desc = nullptr;
     BatchParams bp{};
+    int lanes = 1;                             // 1: batch_kernel; 64: gbatch_kernel
+    GBatchParams gp{};
+    size_t smem = 0;
     uint32_t* d_done = nullptr;
     std::vector<void*> allocs;
     bool ran = false;
@@ -218,9 +383,21 @@ int upload_model(ctl_batch* b, const ctl::Model& m, KParams& p) {
         }
         Dptr[R] = (uint32_t)Dent.size();
     }
+    std::vector<uint32_t> FSptr(S + 1, 0), FSent(m.Fent.size());
+    {
+        std::vector<uint32_t> cnt(S + 1, 0);
+        for (uint32_t e : m.Fent) cnt[(e & 0xffffu) + 1]++;
+        for (int i = 0; i < S; ++i) cnt[i + 1] += cnt[i];
+        for (int i = 0; i <= S; ++i) FSptr[i] = cnt[i];
+        for (int j = 0; j < R; ++j)
+            for (uint32_t h = m.Fptr[j]; h < m.Fptr[j + 1]; ++h) {
+                const uint32_t e = m.Fent[h];
+                FSent[cnt[e & 0xffffu]++] = (uint32_t)j | (e & 0xffff0000u);
+            }
+    }
     std::vector<float> cf(m.c.size());
     for (size_t i = 0; i < m.c.size(); ++i) cf[i] = (float)m.c[i];
-    uint32_t *Lptr, *Lent, *Fptr, *Fent, *Vptr, *Vent, *Dp, *De, *hw, *x0;
+    uint32_t *Lptr, *Lent, *Fptr, *Fent, *Vptr, *Vent, *Dp, *De, *hw, *x0, *FSp, *FSe;
     int32_t* cs;
     double* ts;
     float* cdev;
@@ -231,8 +408,11 @@ int upload_model(ctl_batch* b, const ctl::Model& m, KParams& p) {
         (rc = bupload(b, &Dp, Dptr.data(), Dptr.size())) || (rc = bupload(b, &De, Dent.data(), Dent.size())) ||
         (rc = bupload(b, &hw, horw.data(), horw.size())) || (rc = bupload(b, &cs, m.cs.data(), m.cs.size())) ||
         (rc = bupload(b, &ts, m.ts.data(), m.ts.size())) || (rc = bupload(b, &cdev, cf.data(), cf.size())) ||
-        (rc = bupload(b, &x0, m.x0.data(), m.x0.size())))
+        (rc = bupload(b, &x0, m.x0.data(), m.x0.size())) ||
+        (rc = bupload(b, &FSp, FSptr.data(), FSptr.size())) || (rc = bupload(b, &FSe, FSent.data(), FSent.size())))
         return rc;
+    p.FSptr = FSp;
+    p.FSent = FSe;
     p.S = S; p.R = R; p.K = (int)m.ts.size(); p.SS = (int)m.cs.size();
     p.Lptr = Lptr; p.Lent = Lent; p.Fptr = Fptr; p.Fent = Fent; p.Vptr = Vptr; p.Vent = Vent;
     p.Dptr = Dp; p.Dent = De; p.horw = hw; p.cs = cs; p.ts = ts; p.tmax = m.ts.back();
@@ -283,33 +463,72 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
         return bail(bhip(e, "hipEventCreate"));
     hipDeviceProp_t prop;
     if ((e = hipGetDeviceProperties(&prop, b->device)) != hipSuccess) return bail(bhip(e, "hipGetDeviceProperties"));
-    b->block = o->block_size > 0 ? o->block_size : 256;
-    if (b->block % 64 != 0 || b->block > 256) return bail(ctl::fail(CTL_E_ARGS, "block_size must be 64..256, multiple of 64"));
+    // kernel: one lane per simulation when every model's per-lane working set
+    // fits 16 KB of LDS per wavefront, else one 64-lane wavefront per simulation
+    size_t lane_words = 0, gwords = 0;
+    for (int i = 0; i < nmodels; ++i) {
+        const ctl::Model& m = models[i]->m;
+        lane_words = std::max(lane_words, (size_t)2 * m.S + m.R + (m.c_rows == 1 ? 0 : m.R));
+        gwords = std::max(gwords, (size_t)2 * m.S + (size_t)3 * m.R);
+    }
+    int G = o->lanes_per_sim;
+    if (G == 0) G = lane_words * 4 * 64 <= 16384 ? 1 : 64;
+    if (G != 1 && G != 64) return bail(ctl::fail(CTL_E_ARGS, "a batch runs with lanes_per_sim 0, 1 or 64"));
+    if (G == 64 && gwords * 4 > 65536)
+        return bail(ctl::fail(CTL_E_LIMIT, "a model is too large for the group kernel (2S+3R > 16384)"));
+    b->lanes = G;
     int per_cu = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)batch_kernel, b->block, 0);
-    if (e != hipSuccess || per_cu <= 0) per_cu = 1;
-    int ns = o->slots > 0 ? o->slots : prop.multiProcessorCount * per_cu * b->block;
-    ns = ((ns + b->block - 1) / b->block) * b->block;
-    b->ns = ns;
-    b->grid = ns / b->block;
-
-    // shared slot arrays (sized for the largest model) and counters
+    int rc;
     KParams shared{};
     SlotState& st = shared.st;
-    int rc;
-    int32_t* X;
-    uint32_t* A;
+    int32_t* X = nullptr;
+    uint32_t* A = nullptr;
     float* Cs = nullptr;
-    if ((rc = balloc(b, &X, (size_t)2 * maxS * ns)) || (rc = balloc(b, &A, (size_t)maxR * ns)) ||
-        (any_rates && (rc = balloc(b, &Cs, (size_t)maxR * ns))))
-        return bail(rc);
-    if ((rc = balloc(b, &st.sim, ns)) || (rc = balloc(b, &st.t, ns)) || (rc = balloc(b, &st.tacc, ns)) ||
-        (rc = balloc(b, &st.cursor, ns)) || (rc = balloc(b, &st.mode, ns)) || (rc = balloc(b, &st.evc, ns)) ||
-        (rc = balloc(b, &st.ev, ns)) || (rc = balloc(b, &st.blk, ns)) || (rc = balloc(b, &st.pos, ns)) ||
-        (rc = balloc(b, &st.ssa_left, ns)) || (rc = balloc(b, &st.xb, ns)) || (rc = balloc(b, &st.firings, ns)) ||
-        (rc = balloc(b, &st.ssa_steps, ns)) || (rc = balloc(b, &st.poisson, ns)) || (rc = balloc(b, &st.gaps, ns)) ||
-        (rc = balloc(b, &st.leaps, ns)) || (rc = balloc(b, &st.rejections, ns)) || (rc = balloc(b, &st.switches, ns)))
-        return bail(rc);
+    int ns = 0;
+    if (G == 1) {
+        b->block = o->block_size > 0 ? o->block_size : 256;
+        if (b->block % 64 != 0 || b->block > 256) return bail(ctl::fail(CTL_E_ARGS, "block_size must be 64..256, multiple of 64"));
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)batch_kernel, b->block, 0);
+        if (e != hipSuccess || per_cu <= 0) per_cu = 1;
+        ns = o->slots > 0 ? o->slots : prop.multiProcessorCount * per_cu * b->block;
+        ns = ((ns + b->block - 1) / b->block) * b->block;
+        b->ns = ns;
+        b->grid = ns / b->block;
+        // shared slot arrays (sized for the largest model) and counters
+        if ((rc = balloc(b, &X, (size_t)2 * maxS * ns)) || (rc = balloc(b, &A, (size_t)maxR * ns)) ||
+            (any_rates && (rc = balloc(b, &Cs, (size_t)maxR * ns))))
+            return bail(rc);
+        if ((rc = balloc(b, &st.sim, ns)) || (rc = balloc(b, &st.t, ns)) || (rc = balloc(b, &st.tacc, ns)) ||
+            (rc = balloc(b, &st.cursor, ns)) || (rc = balloc(b, &st.mode, ns)) || (rc = balloc(b, &st.evc, ns)) ||
+            (rc = balloc(b, &st.ev, ns)) || (rc = balloc(b, &st.blk, ns)) || (rc = balloc(b, &st.pos, ns)) ||
+            (rc = balloc(b, &st.ssa_left, ns)) || (rc = balloc(b, &st.xb, ns)) || (rc = balloc(b, &st.firings, ns)) ||
+            (rc = balloc(b, &st.ssa_steps, ns)) || (rc = balloc(b, &st.poisson, ns)) || (rc = balloc(b, &st.gaps, ns)) ||
+            (rc = balloc(b, &st.leaps, ns)) || (rc = balloc(b, &st.rejections, ns)) || (rc = balloc(b, &st.switches, ns)))
+            return bail(rc);
+    } else {
+        b->block = 64;
+        b->smem = gwords * 4;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)gbatch_kernel, 64, b->smem);
+        if (e != hipSuccess || per_cu <= 0) per_cu = 1;
+        long long groups = o->slots > 0 ? (long long)o->slots : (long long)prop.multiProcessorCount * per_cu;
+        if (groups > total) groups = total;
+        b->ns = (int)(groups * 64);
+        b->grid = (int)groups;
+        GBatchParams& g = b->gp;
+        g.ngroups = (int)groups;
+        g.maxS = maxS;
+        g.total = total;
+        if ((rc = balloc(b, &g.slot_model, groups)) || (rc = balloc(b, &g.slot_sim, groups)) ||
+            (rc = balloc(b, &g.t, groups)) || (rc = balloc(b, &g.cursor, groups)) || (rc = balloc(b, &g.evc, groups)) ||
+            (rc = balloc(b, &g.firings, groups)) || (rc = balloc(b, &g.ssa_steps, groups)) ||
+            (rc = balloc(b, &g.leaps, groups)) || (rc = balloc(b, &g.rejections, groups)) ||
+            (rc = balloc(b, &g.switches, groups)) || (rc = balloc(b, &g.poisson, (size_t)groups * 64)) ||
+            (rc = balloc(b, &g.gaps, (size_t)groups * 64)) || (rc = balloc(b, &g.X, (size_t)groups * maxS)) ||
+            (rc = balloc(b, &g.next, 1)))
+            return bail(rc);
+        g.iters = o->iters_per_launch > 0 ? o->iters_per_launch : 65536;
+    }
+    ns = b->ns;
     if ((rc = balloc(b, &b->d_done, 1))) return bail(rc);
     b->desc.resize(nmodels);
     b->reps.assign(replicas, replicas + nmodels);
@@ -354,6 +573,25 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
     std::vector<int> order(nmodels);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] * b->reps[a] > cost[c] * b->reps[c]; });
+    b->stats.lanes_per_sim = G;
+    if (G != 1) {                        // one global counter over the models in cost order
+        std::vector<int64_t> cum(nmodels + 1, 0);
+        for (int k = 0; k < nmodels; ++k) cum[k + 1] = cum[k] + b->reps[order[k]];
+        GBatchParams& g = b->gp;
+        int32_t* d_order;
+        int64_t* d_cum;
+        if ((rc = bupload(b, &d_order, order.data(), order.size())) || (rc = bupload(b, &d_cum, cum.data(), cum.size())))
+            return bail(rc);
+        g.desc = b->d_desc;
+        g.order = d_order;
+        g.cum = d_cum;
+        g.nmodels = nmodels;
+        b->stats.slots = b->ns;
+        b->stats.block_size = 64;
+        b->stats.lds_state = 1;
+        *out = b;
+        return 0;
+    }
     std::vector<int32_t> chunks;
     for (int i : order)
         for (int64_t r = 0; r < b->reps[i]; r += 64) chunks.push_back(i);
@@ -389,9 +627,15 @@ int ctl_batch_run(ctl_batch* b) {
         const KParams& p = b->desc[i];
         BHIP(hipMemsetAsync(p.storage, 0, sizeof(uint32_t) * (size_t)p.n * p.K * p.SS, b->stream), "memset");
     }
-    BHIP(hipMemsetAsync(b->bp.slot_model, 0xff, sizeof(int32_t) * b->ns, b->stream), "memset");
-    BHIP(hipMemsetAsync(b->bp.rep_next, 0, sizeof(uint32_t) * b->nmodels, b->stream), "memset");
-    BHIP(hipMemsetAsync(b->bp.chunk_next, 0, sizeof(uint32_t), b->stream), "memset");
+    if (b->lanes == 1) {
+        BHIP(hipMemsetAsync(b->bp.slot_model, 0xff, sizeof(int32_t) * b->ns, b->stream), "memset");
+        BHIP(hipMemsetAsync(b->bp.rep_next, 0, sizeof(uint32_t) * b->nmodels, b->stream), "memset");
+        BHIP(hipMemsetAsync(b->bp.chunk_next, 0, sizeof(uint32_t), b->stream), "memset");
+    } else {
+        BHIP(hipMemsetAsync(b->gp.slot_sim, 0xff, sizeof(int32_t) * b->gp.ngroups, b->stream), "memset");
+        BHIP(hipMemsetAsync(b->gp.slot_model, 0, sizeof(int32_t) * b->gp.ngroups, b->stream), "memset");
+        BHIP(hipMemsetAsync(b->gp.next, 0, sizeof(uint32_t), b->stream), "memset");
+    }
     BHIP(hipMemsetAsync(b->d_done, 0, sizeof(uint32_t), b->stream), "memset");
     ctl_stats& S = b->stats;
     S.launches = 0;
@@ -399,8 +643,11 @@ int ctl_batch_run(ctl_batch* b) {
     uint32_t done = 0;
     while (done < (uint32_t)b->total) {
         BHIP(hipEventRecord(b->ev0, b->stream), "hipEventRecord");
-        hipLaunchKernelGGL(batch_kernel, dim3(b->grid), dim3(b->block), 0, b->stream, b->bp);
-        BHIP(hipGetLastError(), "batch_kernel launch");
+        if (b->lanes == 1)
+            hipLaunchKernelGGL(batch_kernel, dim3(b->grid), dim3(b->block), 0, b->stream, b->bp);
+        else
+            hipLaunchKernelGGL(gbatch_kernel, dim3(b->grid), dim3(64), b->smem, b->stream, b->gp);
+        BHIP(hipGetLastError(), "batch kernel launch");
         BHIP(hipEventRecord(b->ev1, b->stream), "hipEventRecord");
         BHIP(hipMemcpyAsync(&done, b->d_done, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream), "hipMemcpy D2H");
         BHIP(hipStreamSynchronize(b->stream), "hipStreamSynchronize");

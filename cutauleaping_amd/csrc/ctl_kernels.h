@@ -104,10 +104,12 @@ typedef const __attribute__((address_space(4))) uint32_t* ctab_u32;
 typedef const __attribute__((address_space(4))) int32_t* ctab_i32;
 typedef const __attribute__((address_space(4))) float* ctab_f32;
 typedef const __attribute__((address_space(4))) double* ctab_f64;
+typedef const __attribute__((address_space(4))) int64_t* ctab_i64;
 #define CT_U32(ptr) ((ctab_u32)(ptr))
 #define CT_I32(ptr) ((ctab_i32)(ptr))
 #define CT_F32(ptr) ((ctab_f32)(ptr))
 #define CT_F64(ptr) ((ctab_f64)(ptr))
+#define CT_I64(ptr) ((ctab_i64)(ptr))
 
 __constant__ float c_recip[64] = {
     0.0f,      1.0f,      1.0f / 2,  1.0f / 3,  1.0f / 4,  1.0f / 5,  1.0f / 6,  1.0f / 7,

@@ -159,3 +159,38 @@ def test_group_mass_action_properties_65536(gpu, oracle_lib):
                                  sim_base=int(idx), group=G)
         assert np.array_equal(g[idx], out[0]), idx
         assert gs["firings"][idx] == st["firings"][0]
+
+
+@pytest.mark.parametrize("force", [False, True])
+def test_group_batch_uneven_models_bit_exact(gpu, oracle_lib, force):
+    """Uneven models (S 3..60, R 4..150) and replica counts in one launch of
+    the group batch kernel (one 64-lane wavefront per simulation); each
+    model's samples and counters equal the oracle's group algorithm (G = 64)
+    with that model's stream base; a small iteration budget and few group
+    slots park simulations across launches."""
+    from cutauleaping_amd import networks
+    print(f"case: group batch force_ssa={force}", flush=True)
+    rng = np.random.default_rng(78)
+    models, reps, nets = [], [], []
+    for i in range(10):
+        N = int(rng.integers(3, 61)); M = int(rng.integers(4, 151))
+        net = networks.mass_action(N, M, seed=200 + i, t_end=0.5, check=False)
+        net["x0"] = np.minimum(net["x0"], 3000).astype(np.uint32)
+        ts = np.linspace(0.01, 0.5, int(rng.integers(5, 30)))
+        cs = np.arange(min(N, 6), dtype=np.int32)
+        models.append(gpu.Model.from_arrays(net["left"], net["right"], net["x0"], net["c"], ts, cs=cs))
+        nets.append((net, ts, cs))
+        reps.append(int(rng.integers(1, 40)))
+    with gpu.Batch(models, reps, seed=4343, offset=7, iters_per_launch=40, slots=48, lanes_per_sim=64,
+                   force_ssa=force) as b:
+        b.run()
+        assert b.stats()["lanes_per_sim"] == 64
+        for i, (net, ts, cs) in enumerate(nets):
+            base = b.stream_base(i)
+            out, st = oracle_lib.run(net["left"], net["right"], net["x0"], net["c"], ts, cs=cs, n=reps[i],
+                                     seed=4343, sim_base=7 + base, group=64, force_ssa=force)
+            assert np.array_equal(b.samples(i), out), i
+            gs = b.sim_stats(i)
+            for f in ("firings", "leaps", "rejections", "switches", "ssa_steps", "poisson", "gaps", "events"):
+                assert np.array_equal(gs[f].astype(np.int64), st[f].astype(np.int64)), (i, f)
+        assert b.stats()["simulations"] == sum(reps)

@@ -270,7 +270,7 @@ def test_batched_uneven_models_bit_exact(gpu, oracle_lib):
         models.append(gpu.Model.from_arrays(net["left"], net["right"], net["x0"], net["c"], ts, cs=cs))
         nets.append((net, ts, cs))
         reps.append(int(rng.integers(1, 301)))
-    with gpu.Batch(models, reps, seed=4242, offset=5, iters_per_launch=50, ssa_chunk=4) as b:
+    with gpu.Batch(models, reps, seed=4242, offset=5, iters_per_launch=50, ssa_chunk=4, lanes_per_sim=1) as b:
         b.run()
         for i, (net, ts, cs) in enumerate(nets):
             base = b.stream_base(i)
