@@ -40,6 +40,9 @@ import numpy as np  # noqa: E402
 
 VALU_PEAK = 256 * 64 * 2.4e9          # lane-ops/s (MI355X: 256 CUs, 64 lanes/clk, 2.4 GHz)
 HBM_PEAK = 8.0e12                     # bytes/s
+# HBM bytes per simulation of this workload measured with rocprofv3 FETCH_SIZE / WRITE_SIZE,
+# scaled by a 1 GiB copy (profiles/group_kernel_c1_hbm_traffic.md)
+MEASURED_BYTES_PER_SIM = 81627
 
 
 def lane_ops(stats, S, R, nL, nV, nF, SV):
@@ -75,6 +78,16 @@ def main():
     ap.add_argument("--iters", type=int, default=0)
     ap.add_argument("--ssa-chunk", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per simulation (0: library's choice)")
+    ap.add_argument("--workload", default="c1", choices=("c1", "c2", "c3", "c4"),
+                    help="c1: the headline (default); c2: batched uneven models; c3: surrogate-screened "
+                         "sweep; c4: enzyme cascade (adaptive and force-SSA)")
+    ap.add_argument("--models", type=int, default=1024, help="c2: models in the batch")
+    ap.add_argument("--c3-train", type=int, default=8192)
+    ap.add_argument("--c3-test", type=int, default=512)
+    ap.add_argument("--c3-replicas", type=int, default=128)
+    ap.add_argument("--c3-judge", type=int, default=512)
+    ap.add_argument("--c3-epochs", type=int, default=200)
+    ap.add_argument("--c3-threshold", type=float, default=0.05)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device once (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -85,6 +98,9 @@ def main():
         print("bench.py: this build runs on one GPU only (simulations are independent; "
               "multi-GPU sharding by input offset is documented, not launched).", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "c1":
+        {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[args.workload](args)
+        return
 
     import cutauleaping_amd as ctl
     from cutauleaping_amd import networks
@@ -155,7 +171,10 @@ def main():
                    "samples": info.K, "t_end": args.t_end, "seed": args.seed},
         "simulations_per_s": sim_s,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
-                     "unit": "Tlane-op/s", "frac": achieved / VALU_PEAK, "traffic": None,
+                     "unit": "Tlane-op/s", "frac": achieved / VALU_PEAK,
+                     "traffic": (MEASURED_BYTES_PER_SIM * n / max(last["launches"], 1)
+                                 if last["lanes_per_sim"] > 1 and args.t_end == 50.0 and args.samples == 200 else None),
+                     "traffic_unit": "bytes per launch",
                      "kernel": kname, "avg_launch_ms": avg_launch_s * 1e3,
                      "launches_per_step": last["launches"],
                      "hbm": {"achieved_GBps": hbm_bytes / (ker_ms / 1000.0 / args.steps) / 1e9,
@@ -208,6 +227,105 @@ def cpu_baseline(net, ts, nsims):
             "kind": "port", "sample": f"{nsims} simulations of the same workload (full t_end), "
                                       f"{dt:.1f}s, {st['firings'].sum()} firings",
             "simulations_per_s": nsims / dt}
+
+
+def _line(metric, value, unit, steps, ms, dtype, config, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": 0,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic", "config": config}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def bench_c2(args):
+    """Batched uneven models: networks.batched_models(models, seed=2) -- N
+    log-uniform in 10..500, M in 10..1000, 64..4096 replicas each, rates and
+    counts as C1, 10 sampled species, 100 samples to t_end -- in one
+    persistent launch (library's kernel choice: one 64-lane wavefront per
+    simulation when a model's per-lane state does not fit LDS)."""
+    import cutauleaping_amd as ctl
+    from cutauleaping_amd import networks
+    t_end = args.t_end if args.t_end != 50.0 else 0.5
+    t0 = time.time()
+    nets = networks.batched_models(args.models, seed=2, t_end=t_end, processes=min(16, os.cpu_count() or 1))
+    print(f"# {len(nets)} models generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    ts = networks.sample_times(100, t_end)
+    models = [ctl.Model.from_arrays(n["left"], n["right"], n["x0"], n["c"], ts, cs=n["cs"]) for n in nets]
+    reps = [n["replicas"] for n in nets]
+    fir = 0
+    ms = 0.0
+    with ctl.Batch(models, reps, seed=0xC0FFEE, lanes_per_sim=args.lanes) as b:
+        for k in range(args.steps):
+            t1 = time.perf_counter()
+            b.run()
+            ms += (time.perf_counter() - t1) * 1e3
+            st = b.stats()
+            fir += st["firings"]
+            print(f"# step {k}: firings={st['firings']} kernel_ms={st['kernel_ms']:.1f} launches={st['launches']}",
+                  file=sys.stderr, flush=True)
+    sims = sum(reps)
+    _line("reaction firings/sec on 1 GPU (batched uneven models)", fir / (ms / 1e3), "firings/s", args.steps,
+          ms / args.steps, "f32",
+          {"workload": f"{len(nets)} mass-action models, N 10..500, M 10..1000, {sims} simulations "
+                       f"(64..4096 replicas each), 100 samples to t_end={t_end:g}, one launch sequence",
+           "models": len(nets), "simulations": sims, "t_end": t_end},
+          simulations_per_s=sims * args.steps / (ms / 1e3),
+          counters={k: st[k] for k in ("leaps", "ssa_steps", "switches", "poisson", "gaps", "launches",
+                                       "slots", "lanes_per_sim")},
+          kernel_ms=st["kernel_ms"])
+    for m in models:
+        m.close()
+
+
+def bench_c4(args):
+    """Enzyme cascade: networks.enzyme_cascade(20, seed=4) (61 species, 60
+    reactions), 2^18 simulations, 100 samples to t_end=100, adaptive and
+    force-SSA (two lines)."""
+    import cutauleaping_amd as ctl
+    from cutauleaping_amd import networks
+    net = networks.enzyme_cascade(20, seed=4)
+    t_end = args.t_end if args.t_end != 50.0 else 100.0
+    ts = networks.sample_times(100, t_end)
+    model = ctl.Model.from_arrays(net["left"], net["right"], net["x0"], net["c"], ts)
+    n = min(args.sims, 1 << 18)
+    for force in (False, True):
+        fir, ms = 0, 0.0
+        with ctl.Sim(model, 256, n // 256, seed=0xC0FFEE, force_ssa=force, lanes_per_sim=args.lanes,
+                     verbose=True) as s:
+            for k in range(args.steps):
+                t1 = time.perf_counter()
+                s.run()
+                ms += (time.perf_counter() - t1) * 1e3
+                st = s.stats()
+                fir += st["firings"]
+        _line("reaction firings/sec on 1 GPU (enzyme cascade)", fir / (ms / 1e3), "firings/s", args.steps,
+              ms / args.steps, "f32",
+              {"workload": f"enzyme cascade 20 stages (61 species, 60 reactions), {n} simulations, 100 samples "
+                           f"to t_end={t_end:g}, force_ssa={int(force)}", "simulations": n, "force_ssa": force},
+              simulations_per_s=n * args.steps / (ms / 1e3),
+              counters={k: st[k] for k in ("leaps", "ssa_steps", "switches", "poisson", "gaps", "launches",
+                                           "slots", "lanes_per_sim")},
+              kernel_ms=st["kernel_ms"])
+    model.close()
+
+
+def bench_c3(args):
+    """Surrogate-screened parameter sweep (surrogate.sweep_study): training
+    data from the simulator, a 5-member BF16-MFMA ensemble, then the held-out
+    sweep timed plainly and screened, both judged by the simulator."""
+    from cutauleaping_amd import surrogate as sg
+    prob = sg.SweepProblem.synthetic(N=50, M=100, seed=5, t_end=5.0)
+    say = lambda m: print(m, file=sys.stderr, flush=True)
+    r = sg.sweep_study(prob, n_train=args.c3_train, n_test=args.c3_test, replicas=args.c3_replicas,
+                       judge_replicas=args.c3_judge, members=5, epochs=args.c3_epochs, batch=256, lr=1e-3,
+                       threshold=args.c3_threshold, seed=1, log=say)
+    _line("sweep time speed-up, surrogate-screened vs plain simulator", r["speedup"], "x", 1,
+          r["screened_sweep_s"] * 1e3, "bf16",
+          {"workload": f"mass-action N=50 M=100 seed=5, 32 uncertain rates (log-uniform, 2 decades), 10 observed "
+                       f"species x 50 times to t_end=5; {args.c3_train} training sets x {args.c3_replicas} "
+                       f"replicas; {args.c3_test} held-out sets judged with {r['judge_replicas']} replicas; "
+                       f"MLP 32-256-256-256-1000, 5 members", "n_train": args.c3_train, "n_test": args.c3_test},
+          study=r)
 
 
 if __name__ == "__main__":

@@ -219,9 +219,10 @@ def sweep_study(problem: SweepProblem, *, n_train=8192, n_test=512, replicas=256
     # plain sweep (timed) on the held-out sets; its values are the judge
     jm, js, jst, plain_s = problem.simulate(u_te, judge_replicas, offset=(1 << 30), seed=seed + 1, gpu=gpu,
                                             **sim_opts)
+    pm, ps = jm, js
     if judge_replicas != replicas:
-        _, _, _, plain_s = problem.simulate(u_te, replicas, offset=(1 << 29), seed=seed + 2, gpu=gpu,
-                                            **sim_opts)
+        pm, ps, _, plain_s = problem.simulate(u_te, replicas, offset=(1 << 29), seed=seed + 2, gpu=gpu,
+                                              **sim_opts)
     say(f"# plain sweep of {n_test} sets: {plain_s:.2f}s")
     res = screened_sweep(problem, sur, u_te, replicas=replicas, threshold=threshold, offset=(1 << 28),
                          seed=seed + 3, gpu=gpu, **sim_opts)
@@ -245,4 +246,6 @@ def sweep_study(problem: SweepProblem, *, n_train=8192, n_test=512, replicas=256
         surrogate_only_err_sd=float(relative_error(ss, js).mean()),
         trivial_err_mean=float(relative_error(mm, jm).mean()),
         trivial_err_sd=float(relative_error(ms, js).mean()),
+        plain_err_mean=float(relative_error(pm, jm).mean()) if judge_replicas != replicas else 0.0,
+        plain_err_sd=float(relative_error(ps, js).mean()) if judge_replicas != replicas else 0.0,
     )
