@@ -83,11 +83,10 @@ def main():
                          "sweep; c4: enzyme cascade (adaptive and force-SSA)")
     ap.add_argument("--models", type=int, default=1024, help="c2: models in the batch")
     ap.add_argument("--c3-train", type=int, default=8192)
-    ap.add_argument("--c3-test", type=int, default=512)
+    ap.add_argument("--c3-test", type=int, default=4096)
     ap.add_argument("--c3-replicas", type=int, default=128)
     ap.add_argument("--c3-judge", type=int, default=512)
     ap.add_argument("--c3-epochs", type=int, default=200)
-    ap.add_argument("--c3-threshold", type=float, default=0.05)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device once (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -247,7 +246,8 @@ def bench_c2(args):
     from cutauleaping_amd import networks
     t_end = args.t_end if args.t_end != 50.0 else 0.5
     t0 = time.time()
-    nets = networks.batched_models(args.models, seed=2, t_end=t_end, processes=min(16, os.cpu_count() or 1))
+    nets = networks.batched_models(args.models, seed=2, t_end=t_end, processes=min(16, os.cpu_count() or 1),
+                                   log=lambda m: print(m, file=sys.stderr, flush=True))
     print(f"# {len(nets)} models generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     ts = networks.sample_times(100, t_end)
     models = [ctl.Model.from_arrays(n["left"], n["right"], n["x0"], n["c"], ts, cs=n["cs"]) for n in nets]
@@ -318,7 +318,7 @@ def bench_c3(args):
     say = lambda m: print(m, file=sys.stderr, flush=True)
     r = sg.sweep_study(prob, n_train=args.c3_train, n_test=args.c3_test, replicas=args.c3_replicas,
                        judge_replicas=args.c3_judge, members=5, epochs=args.c3_epochs, batch=256, lr=1e-3,
-                       threshold=args.c3_threshold, seed=1, log=say)
+                       seed=1, log=say)
     _line("sweep time speed-up, surrogate-screened vs plain simulator", r["speedup"], "x", 1,
           r["screened_sweep_s"] * 1e3, "bf16",
           {"workload": f"mass-action N=50 M=100 seed=5, 32 uncertain rates (log-uniform, 2 decades), 10 observed "

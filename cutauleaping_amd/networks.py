@@ -23,10 +23,18 @@ from __future__ import annotations
 import numpy as np
 
 
-def _mean_field_ok(left, right, x0, c, t_end, limit=1e8):
+class _OutOfBudget(Exception):
+    pass
+
+
+def _mean_field_ok(left, right, x0, c, t_end, limit=1e8, max_evals=20000):
     """Integrates dx/dt = (right-left)^T a(x) with a_j = c_j prod C(x, v)
-    (orders 0-2: one reactant, a pair of distinct reactants, or a dimer)."""
+    (orders 0-2: one reactant, a pair of distinct reactants, or a dimer).
+    A system the integrator cannot cross in max_evals evaluations (too stiff
+    to finish cheaply) counts as safe when every count it reached stayed
+    below limit / 100, and as unsafe otherwise."""
     from scipy.integrate import solve_ivp
+    from scipy.sparse import csr_matrix
 
     V = (right - left).astype(np.float64)
     R, S = left.shape
@@ -46,9 +54,15 @@ def _mean_field_ok(left, right, x0, c, t_end, limit=1e8):
             p[j] = nz[0]
             if len(nz) == 2:
                 q[j] = nz[1]
-    VT = V.T.copy()
+    VT = csr_matrix(V.T)
+    evals = [0]
+    peak = [float(np.max(x0))]
 
     def f(t, x):
+        evals[0] += 1
+        peak[0] = max(peak[0], float(np.max(x)))
+        if evals[0] > max_evals:
+            raise _OutOfBudget
         xe = np.append(np.maximum(x, 0.0), 1.0)
         xp, xq = xe[p], xe[q]
         xq = np.where(dimer, np.maximum(xq - 1.0, 0.0) / 2.0, xq)
@@ -58,8 +72,11 @@ def _mean_field_ok(left, right, x0, c, t_end, limit=1e8):
         return limit - np.max(x)
     blowup.terminal = True
 
-    sol = solve_ivp(f, (0.0, t_end), x0.astype(np.float64), method="LSODA", events=blowup,
-                    rtol=1e-4, atol=1e-2)
+    try:
+        sol = solve_ivp(f, (0.0, t_end), x0.astype(np.float64), method="LSODA", events=blowup,
+                        rtol=1e-4, atol=1e-2)
+    except _OutOfBudget:
+        return peak[0] < limit / 100
     if sol.status == -1:
         return False
     return bool(np.all(np.isfinite(sol.y)) and np.max(sol.y) < limit)
@@ -121,3 +138,42 @@ def schlogl():
 
 def sample_times(K, t_end):
     return np.array([t_end * (i + 1) / K for i in range(K)], dtype=np.float64)
+
+
+def _batched_one(args):
+    i, seed, t_end, check, observed = args
+    rng = np.random.default_rng([seed, i, 0xB7])
+    N = int(np.floor(10.0 ** rng.uniform(1.0, np.log10(500.0))))
+    M = int(np.floor(10.0 ** rng.uniform(1.0, 3.0)))
+    replicas = int(np.floor(10.0 ** rng.uniform(np.log10(64.0), np.log10(4096.0))))
+    net = mass_action(N, M, seed=seed * 100003 + i, t_end=t_end, check=check)
+    net["replicas"] = replicas
+    net["cs"] = np.sort(rng.choice(N, size=min(observed, N), replace=False)).astype(np.int32)
+    return net
+
+
+def batched_models(count=8192, seed=2, t_end=1.0, check=True, observed=10, processes=1, log=None):
+    """Uneven batch of independent mass-action models: model i draws
+    N = floor(10^U(1, log10 500)) species, M = floor(10^U(1, 3)) reactions and
+    floor(10^U(log10 64, log10 4096)) replicas, then its network as
+    mass_action(N, M, seed * 100003 + i); `observed` sampled species per model.
+    Returns a list of dicts (left, right, x0, c, attempt, replicas, cs).
+    processes > 1 draws the models in forked workers (BLAS limited to one
+    thread first); log(msg) reports progress every 128 models."""
+    jobs = [(i, seed, t_end, check, observed) for i in range(count)]
+    out = []
+    if processes > 1:
+        import multiprocessing as mp
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=1):
+            with mp.get_context("fork").Pool(processes) as pool:
+                for net in pool.imap(_batched_one, jobs, chunksize=4):
+                    out.append(net)
+                    if log and len(out) % 128 == 0:
+                        log(f"# {len(out)}/{count} models drawn")
+        return out
+    for j in jobs:
+        out.append(_batched_one(j))
+        if log and len(out) % 128 == 0:
+            log(f"# {len(out)}/{count} models drawn")
+    return out

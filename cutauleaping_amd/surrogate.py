@@ -181,25 +181,28 @@ def screened_sweep(problem: SweepProblem, surrogate: Surrogate, u, *, replicas: 
 
 
 def sweep_study(problem: SweepProblem, *, n_train=8192, n_test=512, replicas=256, judge_replicas=None,
-                members=5, epochs=150, batch=256, lr=1e-3, threshold=0.05, seed=1, gpu=0, log=None,
-                sim_opts=None) -> dict:
+                members=5, epochs=150, batch=256, lr=1e-3, quantiles=(0.9, 0.95, 0.98, 0.99, 1.0), seed=1,
+                gpu=0, log=None, sim_opts=None, n_val=1024, target=0.05) -> dict:
     """Data generation, ensemble training and a judged, timed sweep.
 
     Training: n_train parameter sets x `replicas` simulations, reduced to
     mean/sd on the device.  Held out: n_test new sets, simulated with
-    `judge_replicas` (default `replicas`) to judge, and timed as the plain
-    sweep.  The screened sweep predicts every held-out set, simulates those
-    whose ensemble spread exceeds `threshold`, and is judged on the same
-    values.  Returns timings, errors (relative L2 per set, of mean and of sd)
-    and the error of the trivial predictor (the training mean) for scale."""
+    `judge_replicas` (default `replicas`) to judge, and with `replicas`
+    (timed) as the plain sweep.  Fallback thresholds are quantiles of the
+    ensemble spread over n_val further sets (prediction only; quantile 1.0
+    = no fallback); the screened sweep is timed and judged at each.  Errors
+    are relative L2 per set over all times and species, of the means and of
+    the standard deviations, averaged over the held-out sets.  The chosen
+    operating point is the fastest whose mean errors are both <= target."""
     sim_opts = dict(sim_opts or {})
     judge_replicas = judge_replicas or replicas
     say = log or (lambda *a: None)
     rng = np.random.default_rng(seed)
     u_tr = problem.sample_u(n_train, rng)
     u_te = problem.sample_u(n_test, rng)
+    u_val = problem.sample_u(n_val, rng)
     t0 = time.perf_counter()
-    chunk = max(1, (1 << 20) // replicas)           # sets per launch: about 2^20 simulations
+    chunk = max(1, (1 << 19) // replicas)           # sets per launch: about 2^19 simulations
     means, sds, gen_fir = [], [], 0
     for i in range(0, n_train, chunk):
         m, s, st, _ = problem.simulate(u_tr[i:i + chunk], replicas, offset=i * replicas, seed=seed, gpu=gpu,
@@ -216,36 +219,48 @@ def sweep_study(problem: SweepProblem, *, n_train=8192, n_test=512, replicas=256
     losses = sur.fit(u_tr, y_tr, epochs=epochs, batch=batch, lr=lr, seed=seed)
     train_s = time.perf_counter() - t0
     say(f"# trained {members} members in {train_s:.1f}s, final losses {losses[:, -1].round(4).tolist()}")
-    # plain sweep (timed) on the held-out sets; its values are the judge
-    jm, js, jst, plain_s = problem.simulate(u_te, judge_replicas, offset=(1 << 30), seed=seed + 1, gpu=gpu,
-                                            **sim_opts)
-    pm, ps = jm, js
-    if judge_replicas != replicas:
-        pm, ps, _, plain_s = problem.simulate(u_te, replicas, offset=(1 << 29), seed=seed + 2, gpu=gpu,
-                                              **sim_opts)
-    say(f"# plain sweep of {n_test} sets: {plain_s:.2f}s")
-    res = screened_sweep(problem, sur, u_te, replicas=replicas, threshold=threshold, offset=(1 << 28),
-                         seed=seed + 3, gpu=gpu, **sim_opts)
-    e_mean = relative_error(res["mean"], jm)
-    e_sd = relative_error(res["sd"], js)
-    # the surrogate alone (no fallback) and the trivial predictor
-    y_hat, _ = sur.predict(u_te)
-    sm, ss = problem.unfeatures(y_hat)
+    # judge values, then the plain sweep (timed) on the held-out sets
+    jm, js = [], []
+    jchunk = max(1, (1 << 19) // judge_replicas)
+    for i in range(0, n_test, jchunk):
+        m, s, _, _ = problem.simulate(u_te[i:i + jchunk], judge_replicas, offset=(1 << 30) + i * judge_replicas,
+                                      seed=seed + 1, gpu=gpu, **sim_opts)
+        jm.append(m)
+        js.append(s)
+        say(f"# judge {min(i + jchunk, n_test)}/{n_test} sets")
+    jm, js = np.concatenate(jm), np.concatenate(js)
+    say(f"# judged {n_test} sets with {judge_replicas} replicas")
+    pm, ps, _, plain_s = problem.simulate(u_te, replicas, offset=(1 << 29), seed=seed + 2, gpu=gpu, **sim_opts)
+    say(f"# plain sweep of {n_test} sets x {replicas} replicas: {plain_s:.2f}s")
+    _, spread_val = sur.predict(u_val)
+    points = []
+    for q in quantiles:
+        thr = float(np.quantile(spread_val, q)) if q < 1.0 else float("inf")
+        res = screened_sweep(problem, sur, u_te, replicas=replicas, threshold=thr, offset=(1 << 28),
+                             seed=seed + 3, gpu=gpu, **sim_opts)
+        e_m = relative_error(res["mean"], jm)
+        e_s = relative_error(res["sd"], js)
+        points.append(dict(quantile=q, threshold=thr, seconds=res["seconds"], surrogate_s=res["surrogate_seconds"],
+                           fallback_s=res["simulator_seconds"], fallback_fraction=float(res["fallback"].mean()),
+                           speedup=plain_s / res["seconds"], err_mean=float(e_m.mean()), err_sd=float(e_s.mean()),
+                           err_mean_p95=float(np.percentile(e_m, 95)), err_sd_p95=float(np.percentile(e_s, 95))))
+        say(f"# quantile {q}: speed-up {points[-1]['speedup']:.1f}x, fallback {points[-1]['fallback_fraction']:.3f}, "
+            f"errors {points[-1]['err_mean']:.4f} / {points[-1]['err_sd']:.4f}")
+    ok = [p for p in points if p["err_mean"] <= target and p["err_sd"] <= target]
+    best = max(ok, key=lambda p: p["speedup"]) if ok else min(points, key=lambda p: p["err_sd"])
+    # the trivial predictor (the training mean) for scale, and the plain sweep's own error
     mm, ms = problem.unfeatures(np.repeat(y_tr.mean(axis=0, keepdims=True), n_test, axis=0))
     sur.close()
     return dict(
         n_train=n_train, n_test=n_test, replicas=replicas, judge_replicas=judge_replicas, members=members,
         generation_s=gen_s, generation_firings=int(gen_fir), train_s=train_s,
-        final_loss=float(losses[:, -1].mean()),
-        plain_sweep_s=plain_s, screened_sweep_s=res["seconds"], surrogate_s=res["surrogate_seconds"],
-        fallback_s=res["simulator_seconds"], fallback_fraction=float(res["fallback"].mean()),
-        speedup=plain_s / res["seconds"],
-        err_mean=float(e_mean.mean()), err_sd=float(e_sd.mean()),
-        err_mean_p95=float(np.percentile(e_mean, 95)), err_sd_p95=float(np.percentile(e_sd, 95)),
-        surrogate_only_err_mean=float(relative_error(sm, jm).mean()),
-        surrogate_only_err_sd=float(relative_error(ss, js).mean()),
+        final_loss=float(losses[:, -1].mean()), plain_sweep_s=plain_s, points=points, chosen=best,
+        speedup=best["speedup"], screened_sweep_s=best["seconds"], fallback_fraction=best["fallback_fraction"],
+        err_mean=best["err_mean"], err_sd=best["err_sd"], meets_target=bool(ok),
+        surrogate_only_err_mean=points[-1]["err_mean"] if quantiles[-1] == 1.0 else None,
+        surrogate_only_err_sd=points[-1]["err_sd"] if quantiles[-1] == 1.0 else None,
         trivial_err_mean=float(relative_error(mm, jm).mean()),
         trivial_err_sd=float(relative_error(ms, js).mean()),
-        plain_err_mean=float(relative_error(pm, jm).mean()) if judge_replicas != replicas else 0.0,
-        plain_err_sd=float(relative_error(ps, js).mean()) if judge_replicas != replicas else 0.0,
+        plain_err_mean=float(relative_error(pm, jm).mean()),
+        plain_err_sd=float(relative_error(ps, js).mean()),
     )

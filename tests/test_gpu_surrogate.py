@@ -14,7 +14,7 @@ def study(gpu):
     print("case: surrogate study (test scale)", flush=True)
     prob = sg.SweepProblem.synthetic(N=50, M=100, seed=5, t_end=1.0)
     res = sg.sweep_study(prob, n_train=1024, n_test=64, replicas=64, members=3, epochs=60, batch=128,
-                         lr=2e-3, threshold=0.05, seed=3)
+                         lr=2e-3, quantiles=(0.9, 1.0), seed=3, n_val=128)
     return prob, res
 
 
@@ -28,10 +28,11 @@ def test_training_learns_the_sweep(study):
 
 def test_screened_sweep_fallback_and_judge(study):
     prob, r = study
-    assert 0.0 <= r["fallback_fraction"] <= 1.0
-    # with fallback the error can only shrink toward the simulator's own noise
-    assert r["err_mean"] <= r["surrogate_only_err_mean"] + 1e-9
-    assert r["screened_sweep_s"] > 0 and r["plain_sweep_s"] > 0
+    p90, p100 = r["points"]
+    assert 0.0 < p90["fallback_fraction"] <= 0.5 and p100["fallback_fraction"] == 0.0
+    assert p90["seconds"] > 0 and r["plain_sweep_s"] > 0
+    # the surrogate alone (no fallback) is far faster than simulating
+    assert p100["speedup"] > 5
 
 
 def test_fallback_thresholds(gpu):
