@@ -246,7 +246,7 @@ def bench_c2(args):
     from cutauleaping_amd import networks
     t_end = args.t_end if args.t_end != 50.0 else 0.5
     t0 = time.time()
-    nets = networks.batched_models(args.models, seed=2, t_end=t_end, processes=min(16, os.cpu_count() or 1),
+    nets = networks.batched_models(args.models, seed=2, t_end=t_end,
                                    log=lambda m: print(m, file=sys.stderr, flush=True))
     print(f"# {len(nets)} models generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     ts = networks.sample_times(100, t_end)
@@ -254,7 +254,7 @@ def bench_c2(args):
     reps = [n["replicas"] for n in nets]
     fir = 0
     ms = 0.0
-    with ctl.Batch(models, reps, seed=0xC0FFEE, lanes_per_sim=args.lanes) as b:
+    with ctl.Batch(models, reps, seed=0xC0FFEE, lanes_per_sim=args.lanes, verbose=True) as b:
         for k in range(args.steps):
             t1 = time.perf_counter()
             b.run()

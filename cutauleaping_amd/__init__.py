@@ -339,8 +339,9 @@ class Batch:
 
     def __init__(self, models, replicas, *, gpu=0, offset=0, force_ssa=False, seed=0x5EEDC0DE, nc=10,
                  eps=0.03, ssa_steps=100, ssa_factor=10.0, block_size=0, slots=0, iters_per_launch=0,
-                 ssa_chunk=0, lanes_per_sim=0):
+                 ssa_chunk=0, lanes_per_sim=0, verbose=False):
         o = default_opts()
+        o.verbose = int(bool(verbose))
         o.gpu, o.offset, o.force_ssa, o.seed = gpu, offset, int(bool(force_ssa)), seed
         o.nc, o.eps, o.ssa_steps, o.ssa_factor = nc, eps, ssa_steps, ssa_factor
         o.block_size, o.slots, o.iters_per_launch, o.ssa_chunk = block_size, slots, iters_per_launch, ssa_chunk

@@ -310,6 +310,7 @@ struct ctl_batch {
     KParams* d_desc = nullptr;
     BatchParams bp{};
     int lanes = 1;                             // 1: batch_kernel; 64: gbatch_kernel
+    int verbose = 0;
     GBatchParams gp{};
     size_t smem = 0;
     uint32_t* d_done = nullptr;
@@ -454,6 +455,7 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
     ctl_batch* b = new ctl_batch();
     auto bail = [&](int rc) { brelease(b); delete b; return rc; };
     b->device = o->gpu;
+    b->verbose = o->verbose;
     b->nmodels = nmodels;
     b->total = total;
     hipError_t e = hipSetDevice(b->device);
@@ -655,6 +657,9 @@ int ctl_batch_run(ctl_batch* b) {
         BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1), "hipEventElapsedTime");
         total_ms += ms;
         if (++S.launches >= (1 << 20)) return ctl::fail(CTL_E_STATE, "batch did not finish");
+        if (b->verbose)
+            std::fprintf(stderr, " * launch %d: %u/%lld simulations done (%.3f ms)\n", S.launches, done,
+                         (long long)b->total, ms);
     }
     S.kernel_ms = total_ms;
     S.firings = S.leaps = S.rejections = S.switches = S.ssa_steps = S.poisson = S.gaps = S.absorbed = 0;

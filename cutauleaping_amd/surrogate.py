@@ -182,7 +182,7 @@ def screened_sweep(problem: SweepProblem, surrogate: Surrogate, u, *, replicas: 
 
 def sweep_study(problem: SweepProblem, *, n_train=8192, n_test=512, replicas=256, judge_replicas=None,
                 members=5, epochs=150, batch=256, lr=1e-3, quantiles=(0.9, 0.95, 0.98, 0.99, 1.0), seed=1,
-                gpu=0, log=None, sim_opts=None, n_val=1024, target=0.05) -> dict:
+                gpu=0, log=None, sim_opts=None, n_val=1024, target=0.05, policy=0.95) -> dict:
     """Data generation, ensemble training and a judged, timed sweep.
 
     Training: n_train parameter sets x `replicas` simulations, reduced to
@@ -193,7 +193,8 @@ def sweep_study(problem: SweepProblem, *, n_train=8192, n_test=512, replicas=256
     = no fallback); the screened sweep is timed and judged at each.  Errors
     are relative L2 per set over all times and species, of the means and of
     the standard deviations, averaged over the held-out sets.  The chosen
-    operating point is the fastest whose mean errors are both <= target."""
+    operating point is the `policy` quantile (fallback active) when its mean
+    errors are both <= target, else the fastest point that meets the target."""
     sim_opts = dict(sim_opts or {})
     judge_replicas = judge_replicas or replicas
     say = log or (lambda *a: None)
@@ -247,7 +248,8 @@ def sweep_study(problem: SweepProblem, *, n_train=8192, n_test=512, replicas=256
         say(f"# quantile {q}: speed-up {points[-1]['speedup']:.1f}x, fallback {points[-1]['fallback_fraction']:.3f}, "
             f"errors {points[-1]['err_mean']:.4f} / {points[-1]['err_sd']:.4f}")
     ok = [p for p in points if p["err_mean"] <= target and p["err_sd"] <= target]
-    best = max(ok, key=lambda p: p["speedup"]) if ok else min(points, key=lambda p: p["err_sd"])
+    pol = [p for p in ok if p["quantile"] == policy]
+    best = pol[0] if pol else (max(ok, key=lambda p: p["speedup"]) if ok else min(points, key=lambda p: p["err_sd"]))
     # the trivial predictor (the training mean) for scale, and the plain sweep's own error
     mm, ms = problem.unfeatures(np.repeat(y_tr.mean(axis=0, keepdims=True), n_test, axis=0))
     sur.close()
