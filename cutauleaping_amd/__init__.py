@@ -53,6 +53,7 @@ EXPORTS = (
     "ctl_batch_sim_stats", "ctl_batch_stats", "ctl_batch_stream_base", "ctl_batch_destroy",
     "ctl_mlp_create", "ctl_mlp_num_params", "ctl_mlp_params", "ctl_mlp_forward", "ctl_mlp_forward_device",
     "ctl_mlp_gradient", "ctl_mlp_set_data", "ctl_mlp_train", "ctl_mlp_destroy",
+    "ctl_model_load_feeds", "ctl_model_set_feeds",
 )
 
 
@@ -129,6 +130,8 @@ def lib():
         L.ctl_model_info_get.argtypes = [vp, C.POINTER(ctl_model_info)]
         L.ctl_model_structure.argtypes = [vp, vp, vp, vp, vp, vp]
         L.ctl_model_load_fitness.argtypes = [vp, C.c_char_p, i32, i32, C.c_int]
+        L.ctl_model_load_feeds.argtypes = [vp, C.c_char_p, i64, i64, C.c_int]
+        L.ctl_model_set_feeds.argtypes = [vp, vp, i32, vp, i64]
         L.ctl_model_destroy.argtypes = [vp]
         L.ctl_model_destroy.restype = None
         L.ctl_sim_create.argtypes = [vp, C.POINTER(ctl_opts), C.POINTER(vp)]
@@ -232,6 +235,19 @@ class Model:
         _check(lib().ctl_model_structure(self._h, hor.ctypes.data, hort.ctypes.data, lf.ctypes.data,
                                          rf.ctypes.data, ff.ctypes.data))
         return dict(hor=hor, hort=hort, left_flat=lf, rav_flat=rf, fear_flat=ff)
+
+    def load_feeds(self, folder: str, offset: int = 0, n: int = 0) -> None:
+        """Per-simulation feeds from folder/MX_feed (off unless loaded)."""
+        _check(lib().ctl_model_load_feeds(self._h, folder.encode(), offset, n, 0))
+
+    def set_feeds(self, species, values) -> None:
+        """Species species[u] is reset to values[row][u] after every leap and
+        SSA step (rows follow the x0 rule: 1, one per simulation, or groups)."""
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        v = np.ascontiguousarray(np.atleast_2d(values), dtype=np.uint32)
+        if len(sp) and v.shape[1] != len(sp):
+            raise ValueError("values must have one column per feed species")
+        _check(lib().ctl_model_set_feeds(self._h, sp.ctypes.data, len(sp), v.ctypes.data, v.shape[0] if len(sp) else 0))
 
     def load_fitness(self, folder: str, threads: int, blocks: int) -> None:
         _check(lib().ctl_model_load_fitness(self._h, folder.encode(), threads, blocks, 0))

@@ -223,6 +223,7 @@ __global__ __launch_bounds__(64) void gbatch_kernel(GBatchParams bp) {
         L.gg.pos = L.lg.pos = 4;
         for (int s = l; s < p.S; s += G) V.Xb[s] = bp.X[(size_t)slot * bp.maxS + s];
         L.crate = grates(p, L.sim);
+        L.fvrow = p.nfs > 0 ? p.fv + (size_t)(L.sim / p.fv_div) * p.nfs : nullptr;
         gsync();
     }
     for (int it = 0; it < bp.iters; ++it) {
@@ -256,6 +257,7 @@ __global__ __launch_bounds__(64) void gbatch_kernel(GBatchParams bp) {
             for (int s = l; s < p.S; s += G)
                 V.Xb[s] = (int32_t)(p.x0_bcast ? p.x0_bcast[s] : p.x0_sims[(size_t)(r / (uint32_t)p.x0_div) * p.S + s]);
             L.crate = grates(p, (int)r);
+            L.fvrow = p.nfs > 0 ? p.fv + (size_t)(r / (uint32_t)p.fv_div) * p.nfs : nullptr;
             gsync();
         }
         if (p.force_ssa) {
@@ -379,6 +381,9 @@ int upload_model(ctl_batch* b, const ctl::Model& m, KParams& p) {
             for (uint32_t h = m.Fptr[j]; h < m.Fptr[j + 1]; ++h)
                 for (uint32_t r : readers[m.Fent[h] & 0xffffu])
                     if (!mark[r]) { mark[r] = 1; list.push_back(r); }
+            for (int32_t fsp : m.fs)     // feeds reset these species after every SSA step
+                for (uint32_t r : readers[fsp])
+                    if (!mark[r]) { mark[r] = 1; list.push_back(r); }
             std::sort(list.begin(), list.end());
             for (uint32_t r : list) { mark[r] = 0; Dent.push_back(r); }
         }
@@ -414,6 +419,36 @@ int upload_model(ctl_batch* b, const ctl::Model& m, KParams& p) {
         return rc;
     p.FSptr = FSp;
     p.FSent = FSe;
+    p.nfs = (int)m.fs.size();
+    p.fs = nullptr;
+    p.fv = nullptr;
+    if (p.nfs > 0) {
+        int32_t* fsd;
+        uint32_t* fvd;
+        if ((rc = bupload(b, &fsd, m.fs.data(), m.fs.size())) || (rc = bupload(b, &fvd, m.fv.data(), m.fv.size())))
+            return rc;
+        p.fs = fsd;
+        p.fv = fvd;
+    }
+    {   // the same tables packed behind one base pointer (group kernels)
+        std::vector<uint32_t> T;
+        auto put = [&](const uint32_t* d, size_t k) { const int o = (int)T.size(); T.insert(T.end(), d, d + k); return o; };
+        p.oLp = put(m.Lptr.data(), m.Lptr.size());
+        p.oLe = put(m.Lent.data(), m.Lent.size());
+        p.oVp = put(m.Vptr.data(), m.Vptr.size());
+        p.oVe = put(m.Vent.data(), m.Vent.size());
+        p.oHw = put(horw.data(), horw.size());
+        p.oFSp = put(FSptr.data(), FSptr.size());
+        p.oFSe = put(FSent.data(), FSent.size());
+        p.oFp = put(m.Fptr.data(), m.Fptr.size());
+        p.oFe = put(m.Fent.data(), m.Fent.size());
+        p.oDp = put(Dptr.data(), Dptr.size());
+        p.oDe = put(Dent.data(), Dent.size());
+        p.oCs = put(reinterpret_cast<const uint32_t*>(m.cs.data()), m.cs.size());
+        uint32_t* Td;
+        if ((rc = bupload(b, &Td, T.data(), T.size()))) return rc;
+        p.T = Td;
+    }
     p.S = S; p.R = R; p.K = (int)m.ts.size(); p.SS = (int)m.cs.size();
     p.Lptr = Lptr; p.Lent = Lent; p.Fptr = Fptr; p.Fent = Fent; p.Vptr = Vptr; p.Vent = Vent;
     p.Dptr = Dp; p.Dent = De; p.horw = hw; p.cs = cs; p.ts = ts; p.tmax = m.ts.back();
@@ -443,7 +478,8 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
     for (int i = 0; i < nmodels; ++i) {
         if (!models[i] || replicas[i] <= 0) return ctl::fail(CTL_E_ARGS, "null model or non-positive replica count");
         const ctl::Model& m = models[i]->m;
-        if ((m.x0_rows > 1 && m.x0_rows < replicas[i] && replicas[i] % m.x0_rows) ||
+        if ((!m.fs.empty() && m.fv_rows > 1 && m.fv_rows < replicas[i] && replicas[i] % m.fv_rows) ||
+            (m.x0_rows > 1 && m.x0_rows < replicas[i] && replicas[i] % m.x0_rows) ||
             (m.c_rows > 1 && m.c_rows < replicas[i] && replicas[i] % m.c_rows))
             return ctl::fail(CTL_E_ROWS, "a model's input rows must be 1, at least its replicas, or divide them");
         maxS = std::max(maxS, m.S);
@@ -482,7 +518,7 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
     int per_cu = 0;
     int rc;
     KParams shared{};
-    SlotState& st = shared.st;
+    SlotState st{};
     int32_t* X = nullptr;
     uint32_t* A = nullptr;
     float* Cs = nullptr;
@@ -507,6 +543,9 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
             (rc = balloc(b, &st.ssa_steps, ns)) || (rc = balloc(b, &st.poisson, ns)) || (rc = balloc(b, &st.gaps, ns)) ||
             (rc = balloc(b, &st.leaps, ns)) || (rc = balloc(b, &st.rejections, ns)) || (rc = balloc(b, &st.switches, ns)))
             return bail(rc);
+        SlotState* dst;
+        if ((rc = bupload(b, &dst, &st, 1))) return bail(rc);
+        shared.st = dst;
     } else {
         b->block = 64;
         b->smem = gwords * 4;
@@ -546,6 +585,7 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
         p.n = (int)replicas[i];
         p.x0_div = (m.x0_rows <= 1 || m.x0_rows >= p.n) ? 1 : (int)(p.n / m.x0_rows);
         p.c_div = (m.c_rows <= 1 || m.c_rows >= p.n) ? 1 : (int)(p.n / m.c_rows);
+        p.fv_div = m.fv_rows <= 1 ? 0x7fffffff : (m.fv_rows >= p.n ? 1 : (int)(p.n / m.fv_rows));
         b->base[i] = acc;
         p.sim_base = (uint32_t)(o->offset + acc);
         acc += replicas[i];

@@ -114,6 +114,16 @@ int main(int argc, const char* argv[]) {
         rc = ctl_model_load_fitness(model, model_folder, o.threads, o.blocks, verbose);
         if (rc) return die(rc);
     }
+    // Per-simulation feeds are compiled out of the reference (DISABLE_FEED,
+    // T.cu:34); CTL_FEEDS=1 turns them on (read_feeds, T.cu:459-567).
+    const char* feeds_env = std::getenv("CTL_FEEDS");
+    if (feeds_env && std::string(feeds_env) != "0") {
+        rc = ctl_model_load_feeds(model, model_folder, o.offset, n, verbose);
+        if (rc) return die(rc);
+    }
+    // CTL_LANES sets lanes per simulation (default: the library's choice).
+    const char* lanes_env = std::getenv("CTL_LANES");
+    if (lanes_env) o.lanes_per_sim = std::atoi(lanes_env);
     ctl_sim* sim = nullptr;
     rc = ctl_sim_create(model, &o, &sim);
     if (rc) return die(rc);

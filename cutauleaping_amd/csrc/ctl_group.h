@@ -157,6 +157,7 @@ struct GLane {
     uint32_t leaps, rejections, switches, absorbed;
     uint64_t poisson, gaps;
     const float* crate;      // the simulation's rate constants (HBM, cached)
+    const uint32_t* fvrow;   // the simulation's feed values (when the model has feeds)
     Rng gg;                  // group stream (word 3 = 0)
     Rng lg;                  // this lane's stream (word 3 = 1 + lane)
 };
@@ -174,9 +175,9 @@ __device__ __forceinline__ float gprop(const KParams& p, const int32_t* X, const
     float a = crate[j];
     bool c = false;
     const uint32_t nc = (uint32_t)p.nc;
-    const uint32_t h1 = p.Lptr[j + 1];
-    for (uint32_t h = p.Lptr[j]; h < h1; ++h) {
-        const uint32_t e = p.Lent[h];
+    const uint32_t h1 = p.T[p.oLp + j + 1];
+    for (uint32_t h = p.T[p.oLp + j]; h < h1; ++h) {
+        const uint32_t e = p.T[p.oLe + h];
         const int s = (int)(e & 0xffffu);
         const int v = (int)(e >> 16);
         const int32_t x = X[s];
@@ -234,7 +235,7 @@ template <int G>
 __device__ __forceinline__ void gabsorb(const KParams& p, GLane& L, const int32_t* X, int l) {
     uint32_t* out = p.storage + (size_t)L.sim * p.K * p.SS;
     if (L.cursor == 0) {
-        for (int sk = l; sk < p.SS; sk += G) out[sk] = (uint32_t)X[p.cs[sk]];
+        for (int sk = l; sk < p.SS; sk += G) out[sk] = (uint32_t)X[p.T[p.oCs + sk]];
         L.cursor = 1;
     }
     const uint32_t* last = out + (size_t)(L.cursor - 1) * p.SS;
@@ -269,11 +270,11 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
     // tau' (T.cu:1860-1922): the one-lane kernel's per-species values, min over lanes
     float tau1 = 3.40282347e+38f;
     for (int sp = l; sp < p.S; sp += G) {
-        const uint32_t h0 = p.Vptr[sp], h1 = p.Vptr[sp + 1];
+        const uint32_t h0 = p.T[p.oVp + sp], h1 = p.T[p.oVp + sp + 1];
         if (h0 == h1) continue;
         float mu = 0.0f, sg = 0.0f;
         for (uint32_t h = h0; h < h1; ++h) {
-            const uint32_t e = p.Vent[h];
+            const uint32_t e = p.T[p.oVe + h];
             const int j = (int)(e & 0xffffu);
             const int v = (int)(int16_t)(e >> 16);
             const uint32_t u = V.A[j];
@@ -284,7 +285,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         }
         if (mu == 0.0f || sg == 0.0f) continue;
         const int32_t x = X[sp];
-        const float gg = gfun(p.horw[sp], x);
+        const float gg = gfun(p.T[p.oHw + sp], x);
         const float tmp2 = (p.eps * (float)x) / gg;
         const float macs = tmp2 > 1.0f ? tmp2 : 1.0f;
         const float t3 = macs / fabsf(mu);
@@ -368,9 +369,9 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         bool bad = false;
         for (int sp = l; sp < p.S; sp += G) {
             int64_t x = X[sp];
-            const uint32_t h1 = p.FSptr[sp + 1];
-            for (uint32_t h = p.FSptr[sp]; h < h1; ++h) {
-                const uint32_t ent = p.FSent[h];
+            const uint32_t h1 = p.T[p.oFSp + sp + 1];
+            for (uint32_t h = p.T[p.oFSp + sp]; h < h1; ++h) {
+                const uint32_t ent = p.T[p.oFSe + h];
                 x += (int64_t)(int16_t)(ent >> 16) * (int64_t)V.Kc[ent & 0xffffu];
                 if (x < 0 || x > 2147483647LL) { bad = true; break; }
             }
@@ -391,7 +392,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
             while (i < (uint32_t)p.K && tn >= p.ts[i]) {
                 const double ti = p.ts[i];
                 for (int sk = l; sk < p.SS; sk += G) {
-                    const int sp = p.cs[sk];
+                    const int sp = (int)p.T[p.oCs + sk];
                     const int32_t d = X1[sp] - X[sp];
                     uint32_t val = (uint32_t)X[sp];
                     if (d > 0) {
@@ -406,6 +407,11 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         }
     }
     L.xb ^= 1u;
+    if (p.nfs > 0) {                     // feeds (T.cu:2072-2079)
+        int32_t* Xn = V.Xb + (L.xb ? p.S : 0);
+        for (int u = l; u < p.nfs; u += G) Xn[p.fs[u]] = (int32_t)L.fvrow[u];
+        gsync();
+    }
     L.t = L.t + (double)tau;
     L.firings += fired;
     L.leaps++;
@@ -443,25 +449,29 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
         if (i < (uint32_t)p.K && tn > p.ts[i]) {                               // T.cu:2213-2236
             uint32_t* out = p.storage + (size_t)L.sim * p.K * p.SS;
             while (i < (uint32_t)p.K && tn > p.ts[i]) {
-                for (int sk = l; sk < p.SS; sk += G) out[(size_t)i * p.SS + sk] = (uint32_t)X[p.cs[sk]];
+                for (int sk = l; sk < p.SS; sk += G) out[(size_t)i * p.SS + sk] = (uint32_t)X[p.T[p.oCs + sk]];
                 ++i;
             }
             L.cursor = i;
         }
         gsync();
-        const uint32_t f1 = p.Fptr[j + 1];
-        for (uint32_t h = p.Fptr[j] + (uint32_t)l; h < f1; h += G) {
-            const uint32_t ent = p.Fent[h];
+        const uint32_t f1 = p.T[p.oFp + j + 1];
+        for (uint32_t h = p.T[p.oFp + j] + (uint32_t)l; h < f1; h += G) {
+            const uint32_t ent = p.T[p.oFe + h];
             const int s = (int)(ent & 0xffffu);
             const int v = (int)(int16_t)(ent >> 16);
             X[s] = (int32_t)((uint32_t)X[s] + (uint32_t)v);
         }
         gsync();
+        if (p.nfs > 0) {                 // feeds (T.cu:2253-2258); their readers are in the dependency lists
+            for (int u = l; u < p.nfs; u += G) X[p.fs[u]] = (int32_t)L.fvrow[u];
+            gsync();
+        }
         L.ssa_steps++;
         if (tn <= p.tmax) L.firings++;
-        const uint32_t d1 = p.Dptr[j + 1];
-        for (uint32_t d = p.Dptr[j] + (uint32_t)l; d < d1; d += G) {
-            const int r = (int)p.Dent[d];
+        const uint32_t d1 = p.T[p.oDp + j + 1];
+        for (uint32_t d = p.T[p.oDp + j] + (uint32_t)l; d < d1; d += G) {
+            const int r = (int)p.T[p.oDe + d];
             bool crit;
             const float a = gprop(p, X, L.crate, r, crit);
             V.A[r] = __float_as_uint(a) | (crit ? 0x80000000u : 0u);
@@ -496,6 +506,7 @@ __device__ __forceinline__ bool gfetch(const KParams& p, GLane& L, const GView& 
     for (int s = l; s < p.S; s += G)
         V.Xb[s] = (int32_t)(p.x0_bcast ? p.x0_bcast[s] : p.x0_sims[(size_t)(q / (uint32_t)p.x0_div) * p.S + s]);
     L.crate = grates(p, (int)q);
+    L.fvrow = p.nfs > 0 ? p.fv + (size_t)(q / (uint32_t)p.fv_div) * p.nfs : nullptr;
     gsync();
     return true;
 }
@@ -551,26 +562,27 @@ __global__ __launch_bounds__(256) void group_kernel(KParams p) {
     L.gg.k1 = L.lg.k1 = p.key1;
     L.gg.tag = 0u;
     L.lg.tag = 1u + (uint32_t)l;
-    L.sim = p.st.sim[L.slot];
+    L.sim = p.st->sim[L.slot];
     if (L.sim == -2) return;
     if (L.sim >= 0) {                    // resume a parked simulation
         const int s0 = L.slot;
-        L.t = p.st.t[s0];
-        L.cursor = p.st.cursor[s0];
-        L.evc = p.st.evc[s0];
+        L.t = p.st->t[s0];
+        L.cursor = p.st->cursor[s0];
+        L.evc = p.st->evc[s0];
         L.xb = 0;
-        L.firings = p.st.firings[s0];
-        L.ssa_steps = p.st.ssa_steps[s0];
-        L.leaps = p.st.leaps[s0];
-        L.rejections = p.st.rejections[s0];
-        L.switches = p.st.switches[s0];
+        L.firings = p.st->firings[s0];
+        L.ssa_steps = p.st->ssa_steps[s0];
+        L.leaps = p.st->leaps[s0];
+        L.rejections = p.st->rejections[s0];
+        L.switches = p.st->switches[s0];
         L.absorbed = 0;
-        L.poisson = p.st.poisson[(size_t)s0 * G + l];
-        L.gaps = p.st.gaps[(size_t)s0 * G + l];
+        L.poisson = p.st->poisson[(size_t)s0 * G + l];
+        L.gaps = p.st->gaps[(size_t)s0 * G + l];
         L.gg.sim = L.lg.sim = p.sim_base + (uint32_t)L.sim;
         L.gg.pos = L.lg.pos = 4;
         for (int s = l; s < p.S; s += G) V.Xb[s] = p.X[(size_t)s0 * p.S + s];
         L.crate = grates(p, L.sim);
+        L.fvrow = p.nfs > 0 ? p.fv + (size_t)(L.sim / p.fv_div) * p.nfs : nullptr;
         gsync();
     }
     for (int it = 0; it < p.iters; ++it) {
@@ -590,20 +602,20 @@ __global__ __launch_bounds__(256) void group_kernel(KParams p) {
     if (L.sim >= 0) {                    // park: state to the slot arrays
         const int32_t* X = V.Xb + (L.xb ? p.S : 0);
         for (int s = l; s < p.S; s += G) p.X[(size_t)s0 * p.S + s] = X[s];
-        p.st.poisson[(size_t)s0 * G + l] = L.poisson;
-        p.st.gaps[(size_t)s0 * G + l] = L.gaps;
+        p.st->poisson[(size_t)s0 * G + l] = L.poisson;
+        p.st->gaps[(size_t)s0 * G + l] = L.gaps;
         if (l == 0) {
-            p.st.t[s0] = L.t;
-            p.st.cursor[s0] = L.cursor;
-            p.st.evc[s0] = L.evc;
-            p.st.firings[s0] = L.firings;
-            p.st.ssa_steps[s0] = L.ssa_steps;
-            p.st.leaps[s0] = L.leaps;
-            p.st.rejections[s0] = L.rejections;
-            p.st.switches[s0] = L.switches;
+            p.st->t[s0] = L.t;
+            p.st->cursor[s0] = L.cursor;
+            p.st->evc[s0] = L.evc;
+            p.st->firings[s0] = L.firings;
+            p.st->ssa_steps[s0] = L.ssa_steps;
+            p.st->leaps[s0] = L.leaps;
+            p.st->rejections[s0] = L.rejections;
+            p.st->switches[s0] = L.switches;
         }
     }
-    if (l == 0) p.st.sim[s0] = L.sim;
+    if (l == 0) p.st->sim[s0] = L.sim;
 }
 
 }  // namespace ctl

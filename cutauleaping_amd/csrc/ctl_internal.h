@@ -43,6 +43,11 @@ struct Model {
     std::vector<double> ts;                       // K
     std::vector<int32_t> cs;                      // SS
     int64_t row_offset = 0;
+    // per-simulation feeds (read_feeds T.cu:459-567): species fs[u] is reset to
+    // fv[row][u] after every leap and SSA step; none when fs is empty
+    std::vector<int32_t> fs;
+    std::vector<uint32_t> fv;                     // fv_rows * fs.size()
+    int64_t fv_rows = 0;
     // compiled (read_stoichiometry T.cu:122-457)
     int nL = 0, nV = 0, nF = 0;
     std::vector<uint32_t> Lptr, Lent, Fptr, Fent, Vptr, Vent;

@@ -61,6 +61,11 @@ struct KParams {
     const uint32_t* Dent;     //   reads a species changed by reaction j
     const uint32_t* FSptr;    // fear entries species-major, ascending reaction
     const uint32_t* FSent;    //   (group kernel's per-species update)
+    // The u32 tables packed in one buffer, each at a 32-bit word offset: the
+    // group kernels address every table from one base pointer, which keeps
+    // their scalar-register footprint small.
+    const uint32_t* T;
+    int oLp, oLe, oVp, oVe, oHw, oFSp, oFSe, oFp, oFe, oDp, oDe, oCs;
     const uint32_t* horw;     // hor | hort << 8
     const int32_t* cs;
     const double* ts;
@@ -70,6 +75,10 @@ struct KParams {
     const uint32_t* x0_bcast; // S or null
     const uint32_t* x0_sims;  // [row][S] or null; row = sim / x0_div
     int c_div, x0_div;        // simulations per input row (1: one row each)
+    int nfs;                  // per-simulation feeds: species fs[u] reset to
+    const int32_t* fs;        //   fv[sim / fv_div][u] after every leap and SSA step
+    const uint32_t* fv;
+    int fv_div;
     // run
     int n;
     uint32_t sim_base, key0, key1;
@@ -85,7 +94,7 @@ struct KParams {
     int32_t* X;               // [2][S][ns]
     uint32_t* A;              // [R][ns]
     float* Cs;                // [R][ns] when rates differ between simulations
-    SlotState st;
+    const SlotState* st;      // device copy of the slot arrays' pointers (read at launch start / end)
     // outputs
     uint32_t* storage;        // [sim][K][SS]
     ctl_sim_stat* simstat;
@@ -355,6 +364,11 @@ __device__ __forceinline__ int leap(const KParams& p, Lane& L, const View& V, co
         }
     }
     L.xb ^= 1u;                          // X <- X1 (T.cu:2068-2070)
+    if (p.nfs > 0) {                     // feeds (T.cu:2072-2079)
+        int32_t* Xn = xcur(V, L);
+        const uint32_t* fv = p.fv + (size_t)(L.sim / p.fv_div) * p.nfs;
+        for (int u = 0; u < p.nfs; ++u) Xn[(size_t)CT_I32(p.fs)[u] * V.st] = (int32_t)fv[u];
+    }
     L.t = L.t + (double)tau;             // T.cu:2084
     L.firings += fired;
     L.leaps++;
@@ -400,6 +414,10 @@ __device__ __forceinline__ bool ssa_steps(const KParams& p, Lane& L, const View&
             const int v = (int)(int16_t)(e >> 16);
             X[s] = (int32_t)((uint32_t)X[s] + (uint32_t)v);
         }
+        if (p.nfs > 0) {                 // feeds (T.cu:2253-2258); their readers are in the dependency lists
+            const uint32_t* fv = p.fv + (size_t)(L.sim / p.fv_div) * p.nfs;
+            for (int u = 0; u < p.nfs; ++u) X[(size_t)CT_I32(p.fs)[u] * V.st] = (int32_t)fv[u];
+        }
         L.ssa_steps++;
         if (tn <= p.tmax) L.firings++;
         if (st + 1 < steps) {
@@ -419,29 +437,29 @@ __device__ __forceinline__ bool ssa_steps(const KParams& p, Lane& L, const View&
 
 __device__ __forceinline__ void load_lane(const KParams& p, Lane& L) {
     const int s = L.slot;
-    L.sim = p.st.sim[s];
+    L.sim = p.st->sim[s];
     L.g.k0 = p.key0;
     L.g.k1 = p.key1;
     L.absorbed = 0;
     if (L.sim < 0) return;               // idle slot: fetch() initialises the lane
-    L.t = p.st.t[s];
-    L.tacc = p.st.tacc[s];
-    L.cursor = p.st.cursor[s];
-    L.mode = p.st.mode[s];
-    L.evc = p.st.evc[s];
-    L.ssa_left = p.st.ssa_left[s];
-    L.xb = p.st.xb[s];
-    L.firings = p.st.firings[s];
-    L.ssa_steps = p.st.ssa_steps[s];
-    L.poisson = p.st.poisson[s];
-    L.gaps = p.st.gaps[s];
-    L.leaps = p.st.leaps[s];
-    L.rejections = p.st.rejections[s];
-    L.switches = p.st.switches[s];
+    L.t = p.st->t[s];
+    L.tacc = p.st->tacc[s];
+    L.cursor = p.st->cursor[s];
+    L.mode = p.st->mode[s];
+    L.evc = p.st->evc[s];
+    L.ssa_left = p.st->ssa_left[s];
+    L.xb = p.st->xb[s];
+    L.firings = p.st->firings[s];
+    L.ssa_steps = p.st->ssa_steps[s];
+    L.poisson = p.st->poisson[s];
+    L.gaps = p.st->gaps[s];
+    L.leaps = p.st->leaps[s];
+    L.rejections = p.st->rejections[s];
+    L.switches = p.st->switches[s];
     L.g.sim = p.sim_base + (uint32_t)L.sim;
-    L.g.ev = p.st.ev[s];
-    L.g.blk = p.st.blk[s];
-    L.g.pos = p.st.pos[s];
+    L.g.ev = p.st->ev[s];
+    L.g.blk = p.st->blk[s];
+    L.g.pos = p.st->pos[s];
     if (L.g.pos < 4u) {   // regenerate the partially consumed block of an SSA burst
         philox_block(L.g.blk - 1u, L.g.ev, L.g.sim, L.g.tag, L.g.k0, L.g.k1, L.g.b0, L.g.b1, L.g.b2, L.g.b3);
         for (uint32_t i = 0; i < L.g.pos; ++i) {   // drop the words already consumed
@@ -454,25 +472,25 @@ __device__ __forceinline__ void load_lane(const KParams& p, Lane& L) {
 
 __device__ __forceinline__ void store_lane(const KParams& p, const Lane& L) {
     const int s = L.slot;
-    p.st.sim[s] = L.sim;
+    p.st->sim[s] = L.sim;
     if (L.sim < 0) return;
-    p.st.t[s] = L.t;
-    p.st.tacc[s] = L.tacc;
-    p.st.cursor[s] = L.cursor;
-    p.st.mode[s] = L.mode;
-    p.st.evc[s] = L.evc;
-    p.st.ssa_left[s] = L.ssa_left;
-    p.st.xb[s] = L.xb;
-    p.st.firings[s] = L.firings;
-    p.st.ssa_steps[s] = L.ssa_steps;
-    p.st.poisson[s] = L.poisson;
-    p.st.gaps[s] = L.gaps;
-    p.st.leaps[s] = L.leaps;
-    p.st.rejections[s] = L.rejections;
-    p.st.switches[s] = L.switches;
-    p.st.ev[s] = L.g.ev;
-    p.st.blk[s] = L.g.blk;
-    p.st.pos[s] = L.g.pos;
+    p.st->t[s] = L.t;
+    p.st->tacc[s] = L.tacc;
+    p.st->cursor[s] = L.cursor;
+    p.st->mode[s] = L.mode;
+    p.st->evc[s] = L.evc;
+    p.st->ssa_left[s] = L.ssa_left;
+    p.st->xb[s] = L.xb;
+    p.st->firings[s] = L.firings;
+    p.st->ssa_steps[s] = L.ssa_steps;
+    p.st->poisson[s] = L.poisson;
+    p.st->gaps[s] = L.gaps;
+    p.st->leaps[s] = L.leaps;
+    p.st->rejections[s] = L.rejections;
+    p.st->switches[s] = L.switches;
+    p.st->ev[s] = L.g.ev;
+    p.st->blk[s] = L.g.blk;
+    p.st->pos[s] = L.g.pos;
 }
 
 // Per-simulation rates into the lane's rate array (rows that differ).

@@ -414,6 +414,71 @@ int ctl_model_structure(const ctl_model* h, uint8_t* hor, uint8_t* hortype, int3
     return 0;
 }
 
+
+// Per-simulation feeds.  read_feeds (T.cu:459-567): the species whose value
+// is non-zero in the LAST line of MX_feed are the fed species; rows
+// [offset, offset+n) give each simulation's values for them (tab-separated,
+// all S columns).  The reference compiles this out (DISABLE_FEED, T.cu:34),
+// so feeds are only active when a caller loads them.
+int ctl_model_load_feeds(ctl_model* h, const char* dir, int64_t offset, int64_t n, int verbose) {
+    clear_error();
+    if (!h || !dir || offset < 0) return fail(CTL_E_ARGS, "bad arguments");
+    Model& m = h->m;
+    std::vector<std::string> lines;
+    if (!read_lines(std::string(dir) + "/MX_feed", lines) || lines.empty())
+        return fail(CTL_E_NOFILE, "WARNING: cannot read feed quantities");
+    std::vector<std::pair<const char*, const char*>> tok;
+    const std::string& last = lines.back();
+    split_tabs(last.data(), last.data() + last.size(), tok);
+    std::vector<int32_t> fs;
+    for (size_t s2 = 0; s2 < tok.size(); ++s2) {
+        bool ok;
+        if (parse_double(tok[s2].first, tok[s2].second, &ok) != 0.0) fs.push_back((int32_t)s2);
+    }
+    if (fs.empty()) {
+        if (verbose) std::printf(" * No species have feed\n");
+        m.fs.clear(); m.fv.clear(); m.fv_rows = 0;
+        return 0;
+    }
+    int64_t count = n > 0 ? n : (int64_t)lines.size() - offset;
+    if (count <= 0 || offset + count > (int64_t)lines.size())
+        return fail(CTL_E_ROWS, "MX_feed has fewer rows than offset + simulations");
+    std::vector<uint32_t> fv((size_t)count * fs.size());
+    for (int64_t i = 0; i < count; ++i) {
+        const std::string& L = lines[(size_t)(offset + i)];
+        split_tabs(L.data(), L.data() + L.size(), tok);
+        if ((int)tok.size() != m.S) {
+            char b[200];
+            std::snprintf(b, sizeof b, "ERROR! too many quantities %zu (riga %lld)", tok.size(), (long long)i);
+            return fail(CTL_E_WIDTH, b);
+        }
+        for (size_t u = 0; u < fs.size(); ++u) {
+            bool ok;
+            fv[(size_t)i * fs.size() + u] = count_from_double(parse_double(tok[fs[u]].first, tok[fs[u]].second, &ok));
+        }
+    }
+    m.fs = fs;
+    m.fv = fv;
+    m.fv_rows = count;
+    if (verbose) std::printf(" * %zu species have feed\n", fs.size());
+    return 0;
+}
+
+int ctl_model_set_feeds(ctl_model* h, const int32_t* species, int32_t nfs, const uint32_t* values, int64_t rows) {
+    clear_error();
+    if (!h || nfs < 0 || (nfs > 0 && (!species || !values || rows <= 0))) return fail(CTL_E_ARGS, "bad arguments");
+    Model& m = h->m;
+    for (int u = 0; u < nfs; ++u) {
+        if (species[u] < 0 || species[u] >= m.S) return fail(CTL_E_ARGS, "feed species out of range");
+        for (int w = 0; w < u; ++w)
+            if (species[w] == species[u]) return fail(CTL_E_ARGS, "feed species repeated");
+    }
+    m.fs.assign(species, species + nfs);
+    m.fv.assign(values, values + (size_t)(nfs > 0 ? rows : 0) * nfs);
+    m.fv_rows = nfs > 0 ? rows : 0;
+    return 0;
+}
+
 // read_time_series (T.cu:1056-1261).  Missing files only warn in the
 // reference; here p_rep is required (the fitness launch divides by it).
 int ctl_model_load_fitness(ctl_model* h, const char* dir, int32_t threads, int32_t blocks, int verbose) {

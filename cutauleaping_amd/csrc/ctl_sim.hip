@@ -203,6 +203,7 @@ struct ctl_sim {
     bool lds = false;
     size_t smem = 0;
     KParams kp{};
+    SlotState sst{};                 // host copy of the slot arrays' device pointers
     std::vector<void*> allocs;
     bool ran = false;
     ctl_stats stats{};
@@ -389,6 +390,9 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
             for (uint32_t h = m.Fptr[j]; h < m.Fptr[j + 1]; ++h)
                 for (uint32_t r : readers[m.Fent[h] & 0xffffu])
                     if (!mark[r]) { mark[r] = 1; list.push_back(r); }
+            for (int32_t fsp : m.fs)     // feeds reset these species after every SSA step
+                for (uint32_t r : readers[fsp])
+                    if (!mark[r]) { mark[r] = 1; list.push_back(r); }
             std::sort(list.begin(), list.end());
             for (uint32_t r : list) { mark[r] = 0; Dent.push_back(r); }
         }
@@ -420,6 +424,25 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
         return bail(rc);
     p.FSptr = FSp;
     p.FSent = FSe;
+    {   // the same tables packed behind one base pointer (group kernels)
+        std::vector<uint32_t> T;
+        auto put = [&](const uint32_t* d, size_t k) { const int o = (int)T.size(); T.insert(T.end(), d, d + k); return o; };
+        p.oLp = put(m.Lptr.data(), m.Lptr.size());
+        p.oLe = put(m.Lent.data(), m.Lent.size());
+        p.oVp = put(m.Vptr.data(), m.Vptr.size());
+        p.oVe = put(m.Vent.data(), m.Vent.size());
+        p.oHw = put(horw.data(), horw.size());
+        p.oFSp = put(FSptr.data(), FSptr.size());
+        p.oFSe = put(FSent.data(), FSent.size());
+        p.oFp = put(m.Fptr.data(), m.Fptr.size());
+        p.oFe = put(m.Fent.data(), m.Fent.size());
+        p.oDp = put(Dptr.data(), Dptr.size());
+        p.oDe = put(Dent.data(), Dent.size());
+        p.oCs = put(reinterpret_cast<const uint32_t*>(m.cs.data()), m.cs.size());
+        uint32_t* Td;
+        if ((rc = upload(s, &Td, T.data(), T.size()))) return bail(rc);
+        p.T = Td;
+    }
     p.Lptr = Lptr; p.Lent = Lent; p.Fptr = Fptr; p.Fent = Fent; p.Vptr = Vptr; p.Vent = Vent;
     p.Dptr = Dp; p.Dent = De;
     p.horw = hw; p.cs = cs; p.ts = ts; p.tmax = m.ts.back();
@@ -433,6 +456,21 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     p.x0_sims = m.x0_rows == 1 ? nullptr : xdev;
     p.x0_div = (m.x0_rows <= 1 || m.x0_rows >= n) ? 1 : (int)(n / m.x0_rows);
     p.c_div = (m.c_rows <= 1 || m.c_rows >= n) ? 1 : (int)(n / m.c_rows);
+    p.nfs = (int)m.fs.size();
+    p.fs = nullptr;
+    p.fv = nullptr;
+    p.fv_div = 1;
+    if (p.nfs > 0) {
+        if (m.fv_rows > 1 && m.fv_rows < n && n % m.fv_rows)
+            return bail(ctl::fail(CTL_E_ROWS, "MX_feed rows must be 1, at least the simulation count, or divide it"));
+        int32_t* fsd;
+        uint32_t* fvd;
+        if ((rc = upload(s, &fsd, m.fs.data(), m.fs.size())) || (rc = upload(s, &fvd, m.fv.data(), m.fv.size())))
+            return bail(rc);
+        p.fs = fsd;
+        p.fv = fvd;
+        p.fv_div = m.fv_rows <= 1 ? 0x7fffffff : (m.fv_rows >= n ? 1 : (int)(n / m.fv_rows));
+    }
     p.n = n;
     p.sim_base = (uint32_t)o->offset;
     p.key0 = (uint32_t)o->seed;
@@ -452,7 +490,7 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
         if ((rc = dalloc(s, &p.X, (size_t)2 * S * ns)) || (rc = dalloc(s, &p.A, (size_t)R * ns))) return bail(rc);
         if (!p.c_bcast && (rc = dalloc(s, &p.Cs, (size_t)R * ns))) return bail(rc);
     }
-    SlotState& st = p.st;
+    SlotState& st = s->sst;
     if ((rc = dalloc(s, &st.sim, ns)) || (rc = dalloc(s, &st.t, ns)) || (rc = dalloc(s, &st.tacc, ns)) ||
         (rc = dalloc(s, &st.cursor, ns)) || (rc = dalloc(s, &st.mode, ns)) || (rc = dalloc(s, &st.evc, ns)) ||
         (rc = dalloc(s, &st.ev, ns)) || (rc = dalloc(s, &st.blk, ns)) || (rc = dalloc(s, &st.pos, ns)) ||
@@ -461,6 +499,11 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
         (rc = dalloc(s, &st.leaps, ns)) ||
         (rc = dalloc(s, &st.rejections, ns)) || (rc = dalloc(s, &st.switches, ns)))
         return bail(rc);
+    {
+        SlotState* dst;
+        if ((rc = upload(s, &dst, &s->sst, 1))) return bail(rc);
+        p.st = dst;
+    }
     if ((rc = dalloc(s, &p.storage, (size_t)n * K * SS)) || (rc = dalloc(s, &p.simstat, n)) ||
         (rc = dalloc(s, &p.xfinal, (size_t)n * S)) || (rc = dalloc(s, &p.tfinal, n)) ||
         (rc = dalloc(s, &p.next_sim, 2)))
@@ -482,7 +525,7 @@ int ctl_sim_run(ctl_sim* s) {
     const auto w0 = std::chrono::steady_clock::now();
     // PrepareVariables: zero the samples, reset slots and the queue.
     HIPCHK(hipMemsetAsync(p.storage, 0, sizeof(uint32_t) * (size_t)s->n * p.K * p.SS, s->stream), "memset");
-    HIPCHK(hipMemsetAsync(p.st.sim, 0xff, sizeof(int32_t) * s->ns, s->stream), "memset");
+    HIPCHK(hipMemsetAsync(s->sst.sim, 0xff, sizeof(int32_t) * s->ns, s->stream), "memset");
     HIPCHK(hipMemsetAsync(p.next_sim, 0, 2 * sizeof(uint32_t), s->stream), "memset");
     ctl_stats& S = s->stats;
     S.kernel_ms = 0.0;

@@ -151,6 +151,15 @@ int ctl_model_structure(const ctl_model* m, uint8_t* hor, uint8_t* hortype,
  * (read_time_series, T.cu:1056-1261). */
 int ctl_model_load_fitness(ctl_model* m, const char* dir, int32_t threads, int32_t blocks, int verbose);
 
+/* Per-simulation feeds, read_feeds (T.cu:459-567): the species with a
+ * non-zero value in the last line of MX_feed are reset, after every leap and
+ * every SSA step (T.cu:2072-2079, 2253-2258, 2434-2440), to the value in the
+ * simulation's row (rows [offset, offset+n); n <= 0: all from offset).  The
+ * reference compiles feeds out by default (DISABLE_FEED, T.cu:34): a model
+ * has none unless one of these is called.  Rows follow the x0_rows rule. */
+int ctl_model_load_feeds(ctl_model* m, const char* dir, int64_t offset, int64_t n, int verbose);
+int ctl_model_set_feeds(ctl_model* m, const int32_t* species, int32_t nfs, const uint32_t* values, int64_t rows);
+
 void ctl_model_destroy(ctl_model* m);
 
 /* ---- simulation -------------------------------------------------------- */

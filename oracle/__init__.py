@@ -43,7 +43,9 @@ class orc_job(C.Structure):
                 ("sim_base", C.c_uint32),
                 ("nc", C.c_int), ("eps", C.c_float), ("ssa_steps", C.c_int),
                 ("ssa_factor", C.c_float), ("force_ssa", C.c_int),
-                ("precision", C.c_int), ("threads", C.c_int), ("group", C.c_int)]
+                ("precision", C.c_int), ("threads", C.c_int),
+                ("nfs", C.c_int), ("fs", C.POINTER(C.c_int32)), ("fv", C.POINTER(C.c_uint32)), ("fv_rows", C.c_int),
+                ("group", C.c_int)]
 
 
 class orc_stat(C.Structure):
@@ -133,10 +135,11 @@ def compile_model(left, right, locked=None) -> dict:
 
 def run(left, right, x0, c, ts, *, locked=None, cs=None, n=None, seed=1, sim_base=0,
         nc=10, eps=0.03, ssa_steps=100, ssa_factor=10.0, force_ssa=False,
-        precision=REPLAY, threads=0, final=False, group=0):
+        precision=REPLAY, threads=0, final=False, group=0, feed_species=None, feed_values=None):
     """Runs n simulations; returns (samples[n, K, SS] uint32, stats structured
     array) and, with final=True, also the final states [n, S] int32.
-    group=G > 0 runs the group algorithm with G lanes per simulation."""
+    group=G > 0 runs the group algorithm with G lanes per simulation;
+    feed_species / feed_values ([rows, nfs]) enable per-simulation feeds."""
     x0 = np.ascontiguousarray(np.atleast_2d(x0), dtype=np.uint32)
     c = np.ascontiguousarray(np.atleast_2d(c), dtype=np.float64)
     ts = np.ascontiguousarray(ts, dtype=np.float64)
@@ -150,7 +153,13 @@ def run(left, right, x0, c, ts, *, locked=None, cs=None, n=None, seed=1, sim_bas
     job = orc_job(n, _ptr(x0, C.c_uint32), x0.shape[0], _ptr(c, C.c_double), c.shape[0],
                   _ptr(ts, C.c_double), len(ts), _ptr(cs, C.c_int32), len(cs),
                   seed & 0xffffffff, (seed >> 32) & 0xffffffff, sim_base,
-                  nc, eps, ssa_steps, ssa_factor, int(bool(force_ssa)), precision, threads, group)
+                  nc, eps, ssa_steps, ssa_factor, int(bool(force_ssa)), precision, threads,
+                  0, C.POINTER(C.c_int32)(), C.POINTER(C.c_uint32)(), 0, group)
+    if feed_species is not None and len(feed_species):
+        fsa = np.ascontiguousarray(feed_species, dtype=np.int32)
+        fva = np.ascontiguousarray(np.atleast_2d(feed_values), dtype=np.uint32)
+        assert fva.shape[1] == len(fsa)
+        job.nfs, job.fs, job.fv, job.fv_rows = len(fsa), _ptr(fsa, C.c_int32), _ptr(fva, C.c_uint32), fva.shape[0]
     out = np.zeros((n, len(ts), len(cs)), dtype=np.uint32)
     st = np.zeros(n, dtype=STAT_DTYPE)
     assert st.dtype.itemsize == C.sizeof(orc_stat)

@@ -378,6 +378,7 @@ typedef struct {
     double* Ad;
     unsigned char* C;
     uint32_t* out;        /* K*SS samples of this simulation */
+    const uint32_t* fv;   /* feed values of this simulation (nfs) */
     double t, tmax;
     uint32_t cursor, evc;
     int mode;
@@ -493,6 +494,13 @@ static int apply_leap(simctx* s, int first, int pick, float tauf, double taud, i
 
 static void swap_state(simctx* s) { int32_t* tmp = s->X; s->X = s->X1; s->X1 = tmp; }
 
+/* Per-simulation feeds (read_feeds T.cu:459-567, applied after every accepted
+ * leap, T.cu:2072-2079, and after every SSA step, T.cu:2253-2258 and
+ * 2434-2440): each feed species is reset to the simulation's feed value. */
+static void apply_feeds(simctx* s) {
+    for (int u = 0; u < s->J->nfs; ++u) s->X[s->J->fs[u]] = (int32_t)s->fv[u];
+}
+
 /* ======================= REPLAY precision (FP32) ====================== */
 
 /* CalculatePropensity<DEV_FLATTEN_MATRIX_CONST>, T.cu:1443-1470, in FP32;
@@ -603,6 +611,7 @@ static int leap_f32(simctx* s) {
     }
     leap_samples(s, (double)tau);                                      /* T.cu:2065 */
     swap_state(s);                                                     /* T.cu:2068-2070 */
+    apply_feeds(s);
     s->t = s->t + (double)tau;                                         /* T.cu:2084 */
     s->st.firings += fired;
     s->st.leaps++;
@@ -628,6 +637,7 @@ static void ssa_f32(simctx* s) {
         ssa_samples(s, tn);
         for (int h = m->Fptr[j]; h < m->Fptr[j + 1]; ++h)
             s->X[m->Fs[h]] = (int32_t)((uint32_t)s->X[m->Fs[h]] + (uint32_t)m->Fv[h]);
+        apply_feeds(s);
         s->st.ssa_steps++;
         if (tn <= s->tmax) s->st.firings++;
     }
@@ -728,6 +738,7 @@ static int leap_f64(simctx* s) {
     }
     leap_samples(s, tau);
     swap_state(s);
+    apply_feeds(s);
     s->t += tau;
     s->st.firings += fired;
     s->st.leaps++;
@@ -752,6 +763,7 @@ static void ssa_f64(simctx* s) {
         ssa_samples(s, tn);
         for (int h = m->Fptr[j]; h < m->Fptr[j + 1]; ++h)
             s->X[m->Fs[h]] = (int32_t)((uint32_t)s->X[m->Fs[h]] + (uint32_t)m->Fv[h]);
+        apply_feeds(s);
         s->st.ssa_steps++;
         if (tn <= s->tmax) s->st.firings++;
     }
@@ -946,6 +958,7 @@ static int g_leap(grpctx* q) {
     }
     leap_samples(s, (double)tau);
     swap_state(s);
+    apply_feeds(s);
     s->t = s->t + (double)tau;
     s->st.firings += fired;
     s->st.leaps++;
@@ -975,6 +988,7 @@ static void g_ssa(grpctx* q) {
         ssa_samples(s, tn);
         for (int h = m->Fptr[j]; h < m->Fptr[j + 1]; ++h)
             s->X[m->Fs[h]] = (int32_t)((uint32_t)s->X[m->Fs[h]] + (uint32_t)m->Fv[h]);
+        apply_feeds(s);
         s->st.ssa_steps++;
         if (tn <= s->tmax) s->st.firings++;
     }
@@ -1017,6 +1031,11 @@ int orc_run(const orc_net* net, const orc_job* J, uint32_t* storage, orc_stat* s
     if (J->K <= 0 || J->SS < 0 || J->n < 0) return -1;
     if (J->group < 0 || J->group > 64 || (J->group & (J->group - 1))) return -1;
     if (J->group > 0 && J->precision != ORC_REPLAY) return -1;
+    if (J->nfs < 0 || (J->nfs > 0 && (!J->fs || !J->fv || J->fv_rows < 1)) ||
+        (J->nfs > 0 && J->fv_rows > 1 && J->fv_rows < J->n && J->n % J->fv_rows))
+        return -1;
+    for (int u = 0; u < J->nfs; ++u)
+        if (J->fs[u] < 0 || J->fs[u] >= net->S) return -1;
     if ((J->x0_rows > 1 && J->x0_rows < J->n && J->n % J->x0_rows) ||
         (J->c_rows > 1 && J->c_rows < J->n && J->n % J->c_rows))
         return -1;
@@ -1052,6 +1071,7 @@ int orc_run(const orc_net* net, const orc_job* J, uint32_t* storage, orc_stat* s
             const uint32_t* x0 = J->x0 + (size_t)row_of(J->x0_rows, J->n, i) * S;
             for (int sp = 0; sp < S; ++sp) X[sp] = (int32_t)x0[sp];
             s.out = storage + (size_t)i * J->K * J->SS;
+            s.fv = J->nfs > 0 ? J->fv + (size_t)row_of(J->fv_rows, J->n, i) * J->nfs : NULL;
             memset(s.out, 0, sizeof(uint32_t) * (size_t)J->K * J->SS);   /* PrepareVariables T.cu:1269-1297 */
             s.t = 0.0; s.tmax = tmax;
             s.g.key[0] = J->seed_lo; s.g.key[1] = J->seed_hi;

@@ -66,6 +66,9 @@ typedef struct {
     int force_ssa;
     int precision;                  /* ORC_REPLAY | ORC_REFSEM */
     int threads;                    /* OpenMP threads, 0 = runtime default */
+    int nfs;                        /* per-simulation feeds (read_feeds, T.cu:459-567): */
+    const int32_t* fs;              /*   feed species; each is reset to the simulation's */
+    const uint32_t* fv; int fv_rows;/*   value [row][nfs] after every leap and SSA step */
     int group;                      /* 0: one lane per simulation (REPLAY / REFSEM);
                                        G in {2,4,...,64}: the group algorithm with G lanes
                                        per simulation (FP32, REPLAY precision) */

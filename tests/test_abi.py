@@ -124,3 +124,27 @@ def test_no_device_fails_loudly(ctl):
 def test_cli_usage_exit_code(ctl):
     p = subprocess.run([ctl.CLI_PATH, os.path.join(GOLDEN, "schlogl"), "64"], capture_output=True, text=True)
     assert p.returncode == 255 and "please specify" in p.stdout
+
+
+def test_feed_loader(ctl, tmp_path):
+    """MX_feed: the species non-zero in the last line are fed; rows from the
+    offset; width and row errors mirror MX_0's."""
+    import shutil
+    folder = tmp_path / "m"
+    shutil.copytree(os.path.join(GOLDEN, "schlogl"), folder)
+    m = ctl.Model.from_folder(str(folder), 0, 4)
+    with pytest.raises(ctl.CtlError) as e:
+        m.load_feeds(str(folder), 0, 4)
+    assert e.value.code == ctl.CTL_E_NOFILE
+    (folder / "MX_feed").write_text("".join(f"0\t{10 + i}\t0\n" for i in range(4)))
+    m.load_feeds(str(folder), 0, 4)
+    with pytest.raises(ctl.CtlError) as e:
+        m.load_feeds(str(folder), 2, 4)
+    assert e.value.code == ctl.CTL_E_ROWS
+    (folder / "MX_feed").write_text("0\t10\n0\t11\n")
+    with pytest.raises(ctl.CtlError) as e:
+        m.load_feeds(str(folder), 0, 2)
+    assert e.value.code == ctl.CTL_E_WIDTH
+    with pytest.raises(ctl.CtlError):
+        m.set_feeds([5], [[1]])
+    m.set_feeds([], np.zeros((1, 0), np.uint32))
