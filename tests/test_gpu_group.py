@@ -194,3 +194,21 @@ def test_group_batch_uneven_models_bit_exact(gpu, oracle_lib, force):
             for f in ("firings", "leaps", "rejections", "switches", "ssa_steps", "poisson", "gaps", "events"):
                 assert np.array_equal(gs[f].astype(np.int64), st[f].astype(np.int64)), (i, f)
         assert b.stats()["simulations"] == sum(reps)
+
+
+def test_autotune_picks_a_measured_candidate(gpu, tmp_path):
+    """The autotuner times every lanes-per-simulation candidate on a short
+    probe of the model and returns the fastest; the result is memoised on disk."""
+    from cutauleaping_amd import autotune, networks
+    print("case: autotune", flush=True)
+    net = networks.mass_action(100, 200, seed=1)
+    ts = networks.sample_times(200, 50.0)
+    cache = str(tmp_path / "tune.json")
+    res = autotune.tune(net["left"], net["right"], net["x0"], net["c"], ts, probe_sims=8192, probe_fraction=0.02,
+                        candidates=(1, 8, 16, 64), cache_file=cache)
+    assert res["lanes_per_sim"] in (1, 8, 16, 64)
+    assert set(res["scores"]) == {1, 8, 16, 64} and min(res["scores"].values()) > 0
+    assert res["scores"][res["lanes_per_sim"]] == max(res["scores"].values())
+    autotune._memo.clear()
+    again = autotune.tune(net["left"], net["right"], net["x0"], net["c"], ts, cache_file=cache)
+    assert again["lanes_per_sim"] == res["lanes_per_sim"]
