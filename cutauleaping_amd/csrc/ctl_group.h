@@ -171,30 +171,76 @@ struct GView {
 
 // Propensity of reaction j from the group's LDS state; the arithmetic of
 // prop_one (CalculatePropensity, T.cu:1443-1470, in FP32).
+__device__ __forceinline__ void greactant(float& a, bool& c, int32_t x, int v, uint32_t nc) {
+    if (v == 1) {
+        a = a * (float)x;
+    } else if (v == 2) {
+        a = a * ((x < 2) ? 0.0f : (0.5f * (float)x) * (float)(x - 1));
+    } else if (x < v) {
+        a = a * 0.0f;
+    } else {
+        for (int q = 1; q <= v; ++q) a = a * ((float)(x - q + 1) / (float)q);
+    }
+    // 0 < floor(x / v) < nc, as compares: v <= x < nc * v
+    c = c || ((uint32_t)x >= (uint32_t)v && (uint64_t)(uint32_t)x < (uint64_t)nc * (uint32_t)v);
+}
+
+// Reaction j from its own reactant list (the SSA dependents).
 __device__ __forceinline__ float gprop(const KParams& p, const int32_t* X, const float* crate, int j, bool& crit) {
     float a = crate[j];
     bool c = false;
-    const uint32_t nc = (uint32_t)p.nc;
     const uint32_t h1 = p.T[p.oLp + j + 1];
     for (uint32_t h = p.T[p.oLp + j]; h < h1; ++h) {
         const uint32_t e = p.T[p.oLe + h];
-        const int s = (int)(e & 0xffffu);
-        const int v = (int)(e >> 16);
-        const int32_t x = X[s];
-        if (v == 1) {
-            a = a * (float)x;
-        } else if (v == 2) {
-            a = a * ((x < 2) ? 0.0f : (0.5f * (float)x) * (float)(x - 1));
-        } else if (x < v) {
-            a = a * 0.0f;
-        } else {
-            for (int q = 1; q <= v; ++q) a = a * ((float)(x - q + 1) / (float)q);
-        }
-        // 0 < floor(x / v) < nc, as compares: v <= x < nc * v
-        c = c || ((uint32_t)x >= (uint32_t)v && (uint64_t)(uint32_t)x < (uint64_t)nc * (uint32_t)v);
+        greactant(a, c, X[e & 0xffffu], (int)(e >> 16), (uint32_t)p.nc);
     }
     crit = c;
     return a;
+}
+
+// Every reaction of lane l (j = l, l + G, ...) from the reactant round
+// stream: the table addresses do not depend on loaded data, so the loads of a
+// round issue together.  Writes A[j] (propensity bits | critical << 31) and
+// returns the lane's partial sums of all and of the critical propensities.
+template <int G>
+__device__ __forceinline__ void gprops(const KParams& p, const int32_t* X, const float* crate, uint32_t* A, int l,
+                                       float& P, float& PC) {
+    const ctab_u32 TC = CT_U32(p.T);
+    const uint32_t* E = p.T + p.oPRe + l;
+    const int nr = (p.R + G - 1) / G;
+    P = 0.0f;
+    PC = 0.0f;
+    for (int r = 0, j = l; r < nr; ++r, j += G) {
+        const uint32_t len = TC[p.oPRm + r];
+        if (j < p.R) {
+            float a = crate[j];
+            bool c = false;
+            if (p.vmax <= 2) {
+                // branch-free: padding multiplies by 1 and is never critical;
+                // nc * v <= 2^32 - 2 fits the 32-bit compare
+                const uint32_t nc = (uint32_t)p.nc;
+                for (uint32_t k = 0; k < len; ++k) {
+                    const uint32_t e = E[k * G];
+                    const uint32_t v = e >> 16;
+                    const int32_t x = X[e & 0xffffu];
+                    const float xf = (float)x;
+                    const float f2 = (x < 2) ? 0.0f : (0.5f * xf) * (float)(x - 1);
+                    a = a * (v == 1u ? xf : (v == 2u ? f2 : 1.0f));
+                    c = c || ((uint32_t)x >= v && (uint32_t)x < nc * v);
+                }
+            } else {
+                for (uint32_t k = 0; k < len; ++k) {
+                    const uint32_t e = E[k * G];
+                    const int v = (int)(e >> 16);
+                    if (v != 0) greactant(a, c, X[e & 0xffffu], v, (uint32_t)p.nc);
+                }
+            }
+            A[j] = __float_as_uint(a) | (c ? 0x80000000u : 0u);
+            P = P + a;
+            if (c) PC = PC + a;
+        }
+        E += len * G;
+    }
 }
 
 // Selection: among the lanes with a positive own partial (Pown), the first
@@ -253,14 +299,8 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
     if (L.t >= p.tmax) return LEAP_DONE;
     int32_t* X = V.Xb + (L.xb ? p.S : 0);
     int32_t* X1 = V.Xb + (L.xb ? 0 : p.S);
-    float P = 0.0f, PC = 0.0f;
-    for (int j = l; j < p.R; j += G) {
-        bool crit;
-        const float a = gprop(p, X, L.crate, j, crit);
-        V.A[j] = __float_as_uint(a) | (crit ? 0x80000000u : 0u);
-        P = P + a;
-        if (crit) PC = PC + a;
-    }
+    float P, PC;
+    gprops<G>(p, X, L.crate, V.A, l, P, PC);
     gsync();
     P = gscan<G>(P, l);
     PC = gscan<G>(PC, l);
@@ -269,21 +309,23 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
     if (a0 == 0.0f) { gabsorb<G>(p, L, X, l); return LEAP_ABSORB; }
     // tau' (T.cu:1860-1922): the one-lane kernel's per-species values, min over lanes
     float tau1 = 3.40282347e+38f;
-    for (int sp = l; sp < p.S; sp += G) {
-        const uint32_t h0 = p.T[p.oVp + sp], h1 = p.T[p.oVp + sp + 1];
-        if (h0 == h1) continue;
+    const ctab_u32 TC = CT_U32(p.T);
+    const int nrs = (p.S + G - 1) / G;
+    const uint32_t* E = p.T + p.oVRe + l;
+    for (int r = 0, sp = l; r < nrs; ++r, sp += G) {
+        const uint32_t len = TC[p.oVRm + r];
         float mu = 0.0f, sg = 0.0f;
-        for (uint32_t h = h0; h < h1; ++h) {
-            const uint32_t e = p.T[p.oVe + h];
-            const int j = (int)(e & 0xffffu);
+        for (uint32_t k = 0; k < len; ++k) {
+            const uint32_t e = E[k * G];
             const int v = (int)(int16_t)(e >> 16);
-            const uint32_t u = V.A[j];
-            if (u & 0x80000000u) continue;
-            const float a = __uint_as_float(u);
+            const uint32_t u = V.A[e & 0xffffu];
+            // critical reactions and padding (v = 0) add an exact zero
+            const float a = ((u & 0x80000000u) || v == 0) ? 0.0f : __uint_as_float(u);
             mu = mu + (float)v * a;
             sg = sg + (float)(v * v) * a;
         }
-        if (mu == 0.0f || sg == 0.0f) continue;
+        E += len * G;
+        if (mu == 0.0f || sg == 0.0f) continue;      // no var entries (sp >= S included)
         const int32_t x = X[sp];
         const float gg = gfun(p.T[p.oHw + sp], x);
         const float tmp2 = (p.eps * (float)x) / gg;
@@ -330,24 +372,17 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
             bool need = false;
             if (j < p.R) {
                 const uint32_t u = V.A[j];
-                int32_t k = 0;
-                if (u & 0x80000000u) {
-                    k = (attempt == 0 && j == pick) ? 1 : 0;
-                } else {
-                    const float lam = tau * __uint_as_float(u);
-                    if (lam < 1.0f) {
-                        if (lam > 0.0f) {
-                            cum = cum + (double)lam;
-                            while (pos < cum) { ++k; pos = pos + (double)exp_rv(L.lg); L.gaps++; }
-                        }
-                    } else {
-                        need = true;
-                    }
-                }
-                if (!need) {
-                    V.Kc[j] = k;
-                    f += (uint64_t)k;
-                }
+                const bool crit = (u & 0x80000000u) != 0u;
+                const float lam = tau * __uint_as_float(u);
+                need = !crit && !(lam < 1.0f);
+                const bool small = !crit && lam < 1.0f && lam > 0.0f;
+                int32_t k = (crit && attempt == 0 && j == pick) ? 1 : 0;
+                cum = small ? cum + (double)lam : cum;
+                // pos >= cum held before this reaction, so only a lane whose
+                // process mass grew draws gaps
+                while (pos < cum) { ++k; pos = pos + (double)exp_rv(L.lg); L.gaps++; }
+                V.Kc[j] = k;                         // a queued reaction's count is drawn below
+                f += (uint64_t)k;
             }
             const uint64_t m = gballot<G>(need);
             if (need) V.Q[nq + (uint32_t)__popcll(m & ((1ull << l) - 1ull))] = j;
@@ -367,15 +402,21 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         }
         gsync();
         bool bad = false;
-        for (int sp = l; sp < p.S; sp += G) {
-            int64_t x = X[sp];
-            const uint32_t h1 = p.T[p.oFSp + sp + 1];
-            for (uint32_t h = p.T[p.oFSp + sp]; h < h1; ++h) {
-                const uint32_t ent = p.T[p.oFSe + h];
-                x += (int64_t)(int16_t)(ent >> 16) * (int64_t)V.Kc[ent & 0xffffu];
-                if (x < 0 || x > 2147483647LL) { bad = true; break; }
+        const uint32_t* F = p.T + p.oFRe + l;
+        for (int r = 0, sp = l; r < nrs; ++r, sp += G) {
+            const uint32_t len = TC[p.oFRm + r];
+            if (sp < p.S) {
+                int64_t x = X[sp];
+                bool bs = false;
+                for (uint32_t k = 0; k < len; ++k) {   // padding adds 0
+                    const uint32_t ent = F[k * G];
+                    x += (int64_t)(int16_t)(ent >> 16) * (int64_t)V.Kc[ent & 0xffffu];
+                    bs = bs || x < 0 || x > 2147483647LL;
+                }
+                X1[sp] = (int32_t)x;
+                bad = bad || bs;
             }
-            X1[sp] = (int32_t)x;
+            F += len * G;
         }
         fired = gsum64<G>(f);
         const bool any_bad = gballot<G>(bad) != 0;
@@ -426,10 +467,9 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
     int32_t* X = V.Xb + (L.xb ? p.S : 0);
     rng_event(L.gg, L.evc++);
     double tacc = 0.0;
-    for (int j = l; j < p.R; j += G) {
-        bool crit;
-        const float a = gprop(p, X, L.crate, j, crit);
-        V.A[j] = __float_as_uint(a) | (crit ? 0x80000000u : 0u);
+    {
+        float P, PC;
+        gprops<G>(p, X, L.crate, V.A, l, P, PC);
     }
     gsync();
     for (int step = 0; step < p.ssa_steps; ++step) {
@@ -616,6 +656,38 @@ __global__ __launch_bounds__(256) void group_kernel(KParams p) {
         }
     }
     if (l == 0) p.st->sim[s0] = L.sim;
+}
+
+// Host: appends the lane-interleaved round streams (KParams::oPRm ..) of a
+// group kernel with G lanes to the packed table buffer T.
+inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, const Model& m,
+                                 const std::vector<uint32_t>& FSptr, const std::vector<uint32_t>& FSent) {
+    auto rounds = [&](int n, const std::vector<uint32_t>& ptr, const std::vector<uint32_t>& ent, int& om, int& oe) {
+        const int nr = (n + G - 1) / G;
+        std::vector<uint32_t> len((size_t)nr, 0u), e;
+        for (int r = 0; r < nr; ++r) {
+            uint32_t mx = 0;
+            for (int l = 0; l < G && r * G + l < n; ++l) {
+                const int i = r * G + l;
+                if (ptr[i + 1] - ptr[i] > mx) mx = ptr[i + 1] - ptr[i];
+            }
+            len[r] = mx;
+            for (uint32_t k = 0; k < mx; ++k)
+                for (int l = 0; l < G; ++l) {
+                    const int i = r * G + l;
+                    e.push_back(i < n && k < ptr[i + 1] - ptr[i] ? ent[ptr[i] + k] : 0u);
+                }
+        }
+        om = (int)T.size();
+        T.insert(T.end(), len.begin(), len.end());
+        oe = (int)T.size();
+        T.insert(T.end(), e.begin(), e.end());
+    };
+    rounds(m.R, m.Lptr, m.Lent, p.oPRm, p.oPRe);
+    p.vmax = 0;
+    for (uint32_t e : m.Lent) p.vmax = std::max(p.vmax, (int)(e >> 16));
+    rounds(m.S, m.Vptr, m.Vent, p.oVRm, p.oVRe);
+    rounds(m.S, FSptr, FSent, p.oFRm, p.oFRe);
 }
 
 }  // namespace ctl

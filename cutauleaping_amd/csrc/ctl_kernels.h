@@ -66,6 +66,15 @@ struct KParams {
     // their scalar-register footprint small.
     const uint32_t* T;
     int oLp, oLe, oVp, oVe, oHw, oFSp, oFSe, oFp, oFe, oDp, oDe, oCs;
+    // Lane-interleaved round streams of the group kernel, built for its G:
+    // round r covers reactions (or species) r*G .. r*G+G-1, lane l takes the
+    // r*G+l-th; the round holds len = T[o*m + r] entries per lane, entry k of
+    // lane l at T[o*e + base_r + k*G + l] (base_r = G * sum of the earlier
+    // lengths).  Shorter lists are padded with 0 (multiplicity / change 0).
+    // Reactant entries (species | multiplicity << 16), var entries of the
+    // tau' sums and species-major fear entries (reaction | change << 16).
+    int oPRm, oPRe, oVRm, oVRe, oFRm, oFRe;
+    int vmax;                 // largest reactant multiplicity (group kernel fast path when <= 2)
     const uint32_t* horw;     // hor | hort << 8
     const int32_t* cs;
     const double* ts;
