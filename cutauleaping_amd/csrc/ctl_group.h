@@ -205,35 +205,71 @@ __device__ __forceinline__ float gprop(const KParams& p, const int32_t* X, const
 template <int G>
 __device__ __forceinline__ void gprops(const KParams& p, const int32_t* X, const float* crate, uint32_t* A, int l,
                                        float& P, float& PC) {
-    const ctab_u32 TC = CT_U32(p.T);
-    const uint32_t* E = p.T + p.oPRe + l;
     const int nr = (p.R + G - 1) / G;
     P = 0.0f;
     PC = 0.0f;
+    if (p.pfast) {
+        // two entries per reaction; four rounds per batch of loads.  Branch-free
+        // arithmetic: padding multiplies by 1 and is never critical, and
+        // nc * v <= 2^32 - 2 fits the 32-bit compare.
+        const uint32_t* E = p.T + p.oPRe + l;
+        const uint32_t nc = (uint32_t)p.nc;
+        for (int r0 = 0; r0 < nr; r0 += 4) {
+            float cr[4];
+            uint32_t e0[4], e1[4];
+            int32_t x0[4], x1[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = r0 + q, j = l + r * G;
+                cr[q] = 0.0f;
+                e0[q] = e1[q] = 0u;
+                if (r < nr) {
+                    e0[q] = E[(2 * r) * G];
+                    e1[q] = E[(2 * r + 1) * G];
+                    if (j < p.R) cr[q] = crate[j];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                x0[q] = X[e0[q] & 0xffffu];
+                x1[q] = X[e1[q] & 0xffffu];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = r0 + q, j = l + r * G;
+                if (r < nr && j < p.R) {
+                    float a = cr[q];
+                    bool c = false;
+                    const int32_t xs[2] = {x0[q], x1[q]};
+                    const uint32_t vs[2] = {e0[q] >> 16, e1[q] >> 16};
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const int32_t x = xs[k];
+                        const uint32_t v = vs[k];
+                        const float xf = (float)x;
+                        const float f2 = (x < 2) ? 0.0f : (0.5f * xf) * (float)(x - 1);
+                        a = a * (v == 1u ? xf : (v == 2u ? f2 : 1.0f));
+                        c = c || ((uint32_t)x >= v && (uint32_t)x < nc * v);
+                    }
+                    A[j] = __float_as_uint(a) | (c ? 0x80000000u : 0u);
+                    P = P + a;
+                    if (c) PC = PC + a;
+                }
+            }
+        }
+        return;
+    }
+    const ctab_u32 TC = CT_U32(p.T);
+    const uint32_t* E = p.T + p.oPRe + l;
     for (int r = 0, j = l; r < nr; ++r, j += G) {
         const uint32_t len = TC[p.oPRm + r];
         if (j < p.R) {
             float a = crate[j];
             bool c = false;
-            if (p.vmax <= 2) {
-                // branch-free: padding multiplies by 1 and is never critical;
-                // nc * v <= 2^32 - 2 fits the 32-bit compare
-                const uint32_t nc = (uint32_t)p.nc;
-                for (uint32_t k = 0; k < len; ++k) {
-                    const uint32_t e = E[k * G];
-                    const uint32_t v = e >> 16;
-                    const int32_t x = X[e & 0xffffu];
-                    const float xf = (float)x;
-                    const float f2 = (x < 2) ? 0.0f : (0.5f * xf) * (float)(x - 1);
-                    a = a * (v == 1u ? xf : (v == 2u ? f2 : 1.0f));
-                    c = c || ((uint32_t)x >= v && (uint32_t)x < nc * v);
-                }
-            } else {
-                for (uint32_t k = 0; k < len; ++k) {
-                    const uint32_t e = E[k * G];
-                    const int v = (int)(e >> 16);
-                    if (v != 0) greactant(a, c, X[e & 0xffffu], v, (uint32_t)p.nc);
-                }
+            for (uint32_t k = 0; k < len; ++k) {
+                const uint32_t e = E[k * G];
+                const int v = (int)(e >> 16);
+                if (v != 0) greactant(a, c, X[e & 0xffffu], v, (uint32_t)p.nc);
             }
             A[j] = __float_as_uint(a) | (c ? 0x80000000u : 0u);
             P = P + a;
@@ -315,14 +351,20 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
     for (int r = 0, sp = l; r < nrs; ++r, sp += G) {
         const uint32_t len = TC[p.oVRm + r];
         float mu = 0.0f, sg = 0.0f;
-        for (uint32_t k = 0; k < len; ++k) {
-            const uint32_t e = E[k * G];
-            const int v = (int)(int16_t)(e >> 16);
-            const uint32_t u = V.A[e & 0xffffu];
-            // critical reactions and padding (v = 0) add an exact zero
-            const float a = ((u & 0x80000000u) || v == 0) ? 0.0f : __uint_as_float(u);
-            mu = mu + (float)v * a;
-            sg = sg + (float)(v * v) * a;
+        for (uint32_t k = 0; k < len; k += 4) {      // len is a multiple of 4
+            uint32_t e[4], u[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) e[i] = E[(k + i) * G];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) u[i] = V.A[e[i] & 0xffffu];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int v = (int)(int16_t)(e[i] >> 16);
+                // critical reactions and padding (v = 0) add an exact zero
+                const float a = ((u[i] & 0x80000000u) || v == 0) ? 0.0f : __uint_as_float(u[i]);
+                mu = mu + (float)v * a;
+                sg = sg + (float)(v * v) * a;
+            }
         }
         E += len * G;
         if (mu == 0.0f || sg == 0.0f) continue;      // no var entries (sp >= S included)
@@ -408,10 +450,18 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
             if (sp < p.S) {
                 int64_t x = X[sp];
                 bool bs = false;
-                for (uint32_t k = 0; k < len; ++k) {   // padding adds 0
-                    const uint32_t ent = F[k * G];
-                    x += (int64_t)(int16_t)(ent >> 16) * (int64_t)V.Kc[ent & 0xffffu];
-                    bs = bs || x < 0 || x > 2147483647LL;
+                for (uint32_t k = 0; k < len; k += 4) {   // len is a multiple of 4; padding adds 0
+                    uint32_t e[4];
+                    int32_t kc[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) e[i] = F[(k + i) * G];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) kc[i] = V.Kc[e[i] & 0xffffu];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        x += (int64_t)(int16_t)(e[i] >> 16) * (int64_t)kc[i];
+                        bs = bs || x < 0 || x > 2147483647LL;
+                    }
                 }
                 X1[sp] = (int32_t)x;
                 bad = bad || bs;
@@ -662,7 +712,10 @@ __global__ __launch_bounds__(256) void group_kernel(KParams p) {
 // group kernel with G lanes to the packed table buffer T.
 inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, const Model& m,
                                  const std::vector<uint32_t>& FSptr, const std::vector<uint32_t>& FSent) {
-    auto rounds = [&](int n, const std::vector<uint32_t>& ptr, const std::vector<uint32_t>& ent, int& om, int& oe) {
+    // fixed > 0: every round has exactly that many entries per lane; else the
+    // round's longest list rounded up to a multiple of quantum
+    auto rounds = [&](int n, const std::vector<uint32_t>& ptr, const std::vector<uint32_t>& ent, int& om, int& oe,
+                      uint32_t quantum, uint32_t fixed) {
         const int nr = (n + G - 1) / G;
         std::vector<uint32_t> len((size_t)nr, 0u), e;
         for (int r = 0; r < nr; ++r) {
@@ -671,6 +724,7 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
                 const int i = r * G + l;
                 if (ptr[i + 1] - ptr[i] > mx) mx = ptr[i + 1] - ptr[i];
             }
+            mx = fixed ? fixed : (mx + quantum - 1) / quantum * quantum;
             len[r] = mx;
             for (uint32_t k = 0; k < mx; ++k)
                 for (int l = 0; l < G; ++l) {
@@ -683,11 +737,15 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
         oe = (int)T.size();
         T.insert(T.end(), e.begin(), e.end());
     };
-    rounds(m.R, m.Lptr, m.Lent, p.oPRm, p.oPRe);
-    p.vmax = 0;
-    for (uint32_t e : m.Lent) p.vmax = std::max(p.vmax, (int)(e >> 16));
-    rounds(m.S, m.Vptr, m.Vent, p.oVRm, p.oVRe);
-    rounds(m.S, FSptr, FSent, p.oFRm, p.oFRe);
+    p.pfast = 1;
+    for (int j = 0; j < m.R; ++j) {
+        if (m.Lptr[j + 1] - m.Lptr[j] > 2) p.pfast = 0;
+        for (uint32_t h = m.Lptr[j]; h < m.Lptr[j + 1]; ++h)
+            if ((m.Lent[h] >> 16) > 2) p.pfast = 0;
+    }
+    rounds(m.R, m.Lptr, m.Lent, p.oPRm, p.oPRe, 1, p.pfast ? 2 : 0);
+    rounds(m.S, m.Vptr, m.Vent, p.oVRm, p.oVRe, 4, 0);
+    rounds(m.S, FSptr, FSent, p.oFRm, p.oFRe, 4, 0);
 }
 
 }  // namespace ctl

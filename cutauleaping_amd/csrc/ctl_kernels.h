@@ -70,11 +70,13 @@ struct KParams {
     // round r covers reactions (or species) r*G .. r*G+G-1, lane l takes the
     // r*G+l-th; the round holds len = T[o*m + r] entries per lane, entry k of
     // lane l at T[o*e + base_r + k*G + l] (base_r = G * sum of the earlier
-    // lengths).  Shorter lists are padded with 0 (multiplicity / change 0).
+    // lengths).  Shorter lists are padded with 0 (multiplicity / change 0);
+    // var and fear round lengths are multiples of 4 (loads issue 4 at a time).
     // Reactant entries (species | multiplicity << 16), var entries of the
     // tau' sums and species-major fear entries (reaction | change << 16).
     int oPRm, oPRe, oVRm, oVRe, oFRm, oFRe;
-    int vmax;                 // largest reactant multiplicity (group kernel fast path when <= 2)
+    int pfast;                // every reaction has <= 2 reactant entries of multiplicity <= 2:
+                              //   the reactant stream then has exactly 2 entries per round
     const uint32_t* horw;     // hor | hort << 8
     const int32_t* cs;
     const double* ts;
