@@ -445,7 +445,7 @@ int upload_model(ctl_batch* b, const ctl::Model& m, KParams& p) {
         p.oDp = put(Dptr.data(), Dptr.size());
         p.oDe = put(Dent.data(), Dent.size());
         p.oCs = put(reinterpret_cast<const uint32_t*>(m.cs.data()), m.cs.size());
-        append_group_streams(T, p, 64, m, FSptr, FSent);
+        append_group_streams(T, p, 64, m, FSptr, FSent, Dent);
         uint32_t* Td;
         if ((rc = bupload(b, &Td, T.data(), T.size()))) return rc;
         p.T = Td;

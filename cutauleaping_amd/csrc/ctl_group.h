@@ -185,6 +185,21 @@ __device__ __forceinline__ void greactant(float& a, bool& c, int32_t x, int v, u
     c = c || ((uint32_t)x >= (uint32_t)v && (uint64_t)(uint32_t)x < (uint64_t)nc * (uint32_t)v);
 }
 
+// The same for a reaction with two reactant entries of multiplicity <= 2
+// (padding: multiplicity 0), branch-free: padding multiplies by 1 and is
+// never critical, and nc * v <= 2^32 - 2 fits the 32-bit compare.
+__device__ __forceinline__ void gfast2(float& a, bool& c, const int32_t* xs, const uint32_t* vs, uint32_t nc) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int32_t x = xs[k];
+        const uint32_t v = vs[k];
+        const float xf = (float)x;
+        const float f2 = (x < 2) ? 0.0f : (0.5f * xf) * (float)(x - 1);
+        a = a * (v == 1u ? xf : (v == 2u ? f2 : 1.0f));
+        c = c || ((uint32_t)x >= v && (uint32_t)x < nc * v);
+    }
+}
+
 // Reaction j from its own reactant list (the SSA dependents).
 __device__ __forceinline__ float gprop(const KParams& p, const int32_t* X, const float* crate, int j, bool& crit) {
     float a = crate[j];
@@ -209,9 +224,7 @@ __device__ __forceinline__ void gprops(const KParams& p, const int32_t* X, const
     P = 0.0f;
     PC = 0.0f;
     if (p.pfast) {
-        // two entries per reaction; four rounds per batch of loads.  Branch-free
-        // arithmetic: padding multiplies by 1 and is never critical, and
-        // nc * v <= 2^32 - 2 fits the 32-bit compare.
+        // two entries per reaction; four rounds per batch of loads
         const uint32_t* E = p.T + p.oPRe + l;
         const uint32_t nc = (uint32_t)p.nc;
         for (int r0 = 0; r0 < nr; r0 += 4) {
@@ -242,15 +255,7 @@ __device__ __forceinline__ void gprops(const KParams& p, const int32_t* X, const
                     bool c = false;
                     const int32_t xs[2] = {x0[q], x1[q]};
                     const uint32_t vs[2] = {e0[q] >> 16, e1[q] >> 16};
-#pragma unroll
-                    for (int k = 0; k < 2; ++k) {
-                        const int32_t x = xs[k];
-                        const uint32_t v = vs[k];
-                        const float xf = (float)x;
-                        const float f2 = (x < 2) ? 0.0f : (0.5f * xf) * (float)(x - 1);
-                        a = a * (v == 1u ? xf : (v == 2u ? f2 : 1.0f));
-                        c = c || ((uint32_t)x >= v && (uint32_t)x < nc * v);
-                    }
+                    gfast2(a, c, xs, vs, nc);
                     A[j] = __float_as_uint(a) | (c ? 0x80000000u : 0u);
                     P = P + a;
                     if (c) PC = PC + a;
@@ -300,13 +305,23 @@ __device__ __forceinline__ int gselect(const KParams& p, const uint32_t* A, floa
     if (l == sel) {
         float acc = l > 0 ? excl : 0.0f;
         int last = -1;
-        for (int j = l; j < p.R; j += G) {
-            const uint32_t u = A[j];
-            if (which == 2 && !((u & 0x80000000u) || j == 0)) continue;
-            const float a = aval(u);
-            acc = acc + a;
-            if (a > 0.0f) last = j;
-            if (w <= acc && a > 0.0f) { pick = j; break; }
+        // four propensities per batch of loads; past the end reads as 0,
+        // which neither moves acc nor can be chosen
+        for (int j0 = l; j0 < p.R && pick < 0; j0 += 4 * G) {
+            uint32_t u[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) u[i] = j0 + i * G < p.R ? A[j0 + i * G] : 0u;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int j = j0 + i * G;
+                const bool use = which != 2 || (u[i] & 0x80000000u) || j == 0;
+                const float a = use ? aval(u[i]) : 0.0f;
+                acc = acc + a;
+                if (pick < 0 && a > 0.0f) {
+                    last = j;
+                    if (w <= acc) pick = j;
+                }
+            }
         }
         if (pick < 0) pick = last;
     }
@@ -524,7 +539,13 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
     gsync();
     for (int step = 0; step < p.ssa_steps; ++step) {
         float P = 0.0f;
-        for (int j = l; j < p.R; j += G) P = P + aval(V.A[j]);
+        for (int j0 = l; j0 < p.R; j0 += 4 * G) {      // past the end adds an exact 0
+            uint32_t u[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) u[i] = j0 + i * G < p.R ? V.A[j0 + i * G] : 0u;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) P = P + aval(u[i]);
+        }
         const float Pown = P;
         P = gscan<G>(P, l);
         const float a0 = gbcast_last<G>(P, l);
@@ -559,12 +580,28 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
         }
         L.ssa_steps++;
         if (tn <= p.tmax) L.firings++;
-        const uint32_t d1 = p.T[p.oDp + j + 1];
-        for (uint32_t d = p.T[p.oDp + j] + (uint32_t)l; d < d1; d += G) {
-            const int r = (int)p.T[p.oDe + d];
-            bool crit;
-            const float a = gprop(p, X, L.crate, r, crit);
-            V.A[r] = __float_as_uint(a) | (crit ? 0x80000000u : 0u);
+        const uint32_t d0 = p.T[p.oDp + j], d1 = p.T[p.oDp + j + 1];
+        if (p.pfast) {
+            // dependent records {r, reactant entry 0, entry 1, 0}: one 16-byte load each
+            const uint4* DR = reinterpret_cast<const uint4*>(p.T + p.oDR);
+            const uint32_t nc = (uint32_t)p.nc;
+            for (uint32_t d = d0 + (uint32_t)l; d < d1; d += G) {
+                const uint4 rec = DR[d];
+                const int r = (int)rec.x;
+                float a = L.crate[r];
+                const int32_t xs[2] = {X[rec.y & 0xffffu], X[rec.z & 0xffffu]};
+                const uint32_t vs[2] = {rec.y >> 16, rec.z >> 16};
+                bool c = false;
+                gfast2(a, c, xs, vs, nc);
+                V.A[r] = __float_as_uint(a) | (c ? 0x80000000u : 0u);
+            }
+        } else {
+            for (uint32_t d = d0 + (uint32_t)l; d < d1; d += G) {
+                const int r = (int)p.T[p.oDe + d];
+                bool crit;
+                const float a = gprop(p, X, L.crate, r, crit);
+                V.A[r] = __float_as_uint(a) | (crit ? 0x80000000u : 0u);
+            }
         }
         gsync();
     }
@@ -711,7 +748,8 @@ __global__ __launch_bounds__(256) void group_kernel(KParams p) {
 // Host: appends the lane-interleaved round streams (KParams::oPRm ..) of a
 // group kernel with G lanes to the packed table buffer T.
 inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, const Model& m,
-                                 const std::vector<uint32_t>& FSptr, const std::vector<uint32_t>& FSent) {
+                                 const std::vector<uint32_t>& FSptr, const std::vector<uint32_t>& FSent,
+                                 const std::vector<uint32_t>& Dent) {
     // fixed > 0: every round has exactly that many entries per lane; else the
     // round's longest list rounded up to a multiple of quantum
     auto rounds = [&](int n, const std::vector<uint32_t>& ptr, const std::vector<uint32_t>& ent, int& om, int& oe,
@@ -744,6 +782,18 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
             if ((m.Lent[h] >> 16) > 2) p.pfast = 0;
     }
     rounds(m.R, m.Lptr, m.Lent, p.oPRm, p.oPRe, 1, p.pfast ? 2 : 0);
+    p.oDR = 0;
+    if (p.pfast) {                        // dependent records, 16-byte aligned
+        while (T.size() % 4) T.push_back(0u);
+        p.oDR = (int)T.size();
+        for (uint32_t r : Dent) {
+            const uint32_t n = m.Lptr[r + 1] - m.Lptr[r];
+            T.push_back(r);
+            T.push_back(n > 0 ? m.Lent[m.Lptr[r]] : 0u);
+            T.push_back(n > 1 ? m.Lent[m.Lptr[r] + 1] : 0u);
+            T.push_back(0u);
+        }
+    }
     rounds(m.S, m.Vptr, m.Vent, p.oVRm, p.oVRe, 4, 0);
     rounds(m.S, FSptr, FSent, p.oFRm, p.oFRe, 4, 0);
 }

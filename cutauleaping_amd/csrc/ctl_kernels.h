@@ -75,6 +75,7 @@ struct KParams {
     // Reactant entries (species | multiplicity << 16), var entries of the
     // tau' sums and species-major fear entries (reaction | change << 16).
     int oPRm, oPRe, oVRm, oVRe, oFRm, oFRe;
+    int oDR;                  // pfast: SSA dependent records {r, entry 0, entry 1, 0} (uint4, Dptr order)
     int pfast;                // every reaction has <= 2 reactant entries of multiplicity <= 2:
                               //   the reactant stream then has exactly 2 entries per round
     const uint32_t* horw;     // hor | hort << 8

@@ -439,7 +439,7 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
         p.oDp = put(Dptr.data(), Dptr.size());
         p.oDe = put(Dent.data(), Dent.size());
         p.oCs = put(reinterpret_cast<const uint32_t*>(m.cs.data()), m.cs.size());
-        if (s->lanes > 1) append_group_streams(T, p, s->lanes, m, FSptr, FSent);
+        if (s->lanes > 1) append_group_streams(T, p, s->lanes, m, FSptr, FSent, Dent);
         uint32_t* Td;
         if ((rc = upload(s, &Td, T.data(), T.size()))) return bail(rc);
         p.T = Td;
