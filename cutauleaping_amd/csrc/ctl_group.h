@@ -363,8 +363,10 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
     const ctab_u32 TC = CT_U32(p.T);
     const int nrs = (p.S + G - 1) / G;
     const uint32_t* E = p.T + p.oVRe + l;
-    for (int r = 0, sp = l; r < nrs; ++r, sp += G) {
+    const uint32_t* SPI = p.T + p.oSPI + l;
+    for (int r = 0; r < nrs; ++r) {
         const uint32_t len = TC[p.oVRm + r];
+        const int sp = (int)SPI[r * G];
         float mu = 0.0f, sg = 0.0f;
         for (uint32_t k = 0; k < len; k += 4) {      // len is a multiple of 4
             uint32_t e[4], u[4];
@@ -382,7 +384,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
             }
         }
         E += len * G;
-        if (mu == 0.0f || sg == 0.0f) continue;      // no var entries (sp >= S included)
+        if (mu == 0.0f || sg == 0.0f) continue;      // no var entries (no species included)
         const int32_t x = X[sp];
         const float gg = gfun(p.T[p.oHw + sp], x);
         const float tmp2 = (p.eps * (float)x) / gg;
@@ -460,9 +462,10 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         gsync();
         bool bad = false;
         const uint32_t* F = p.T + p.oFRe + l;
-        for (int r = 0, sp = l; r < nrs; ++r, sp += G) {
+        for (int r = 0; r < nrs; ++r) {
             const uint32_t len = TC[p.oFRm + r];
-            if (sp < p.S) {
+            const uint32_t sp = SPI[r * G];
+            if (sp < (uint32_t)p.S) {
                 int64_t x = X[sp];
                 bool bs = false;
                 for (uint32_t k = 0; k < len; k += 4) {   // len is a multiple of 4; padding adds 0
@@ -750,24 +753,23 @@ __global__ __launch_bounds__(256) void group_kernel(KParams p) {
 inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, const Model& m,
                                  const std::vector<uint32_t>& FSptr, const std::vector<uint32_t>& FSent,
                                  const std::vector<uint32_t>& Dent) {
-    // fixed > 0: every round has exactly that many entries per lane; else the
-    // round's longest list rounded up to a multiple of quantum
-    auto rounds = [&](int n, const std::vector<uint32_t>& ptr, const std::vector<uint32_t>& ent, int& om, int& oe,
-                      uint32_t quantum, uint32_t fixed) {
-        const int nr = (n + G - 1) / G;
+    // Item order[r * G + l] (-1: none) goes to lane l of round r.  fixed > 0:
+    // every round has exactly that many entries per lane; else the round's
+    // longest list rounded up to a multiple of quantum.
+    auto rounds = [&](const std::vector<int>& order, const std::vector<uint32_t>& ptr,
+                      const std::vector<uint32_t>& ent, int& om, int& oe, uint32_t quantum, uint32_t fixed) {
+        const int nr = (int)(order.size() / (size_t)G);
+        auto cnt = [&](int i) { return i < 0 ? 0u : ptr[i + 1] - ptr[i]; };
         std::vector<uint32_t> len((size_t)nr, 0u), e;
         for (int r = 0; r < nr; ++r) {
             uint32_t mx = 0;
-            for (int l = 0; l < G && r * G + l < n; ++l) {
-                const int i = r * G + l;
-                if (ptr[i + 1] - ptr[i] > mx) mx = ptr[i + 1] - ptr[i];
-            }
+            for (int l = 0; l < G; ++l) mx = std::max(mx, cnt(order[(size_t)r * G + l]));
             mx = fixed ? fixed : (mx + quantum - 1) / quantum * quantum;
             len[r] = mx;
             for (uint32_t k = 0; k < mx; ++k)
                 for (int l = 0; l < G; ++l) {
-                    const int i = r * G + l;
-                    e.push_back(i < n && k < ptr[i + 1] - ptr[i] ? ent[ptr[i] + k] : 0u);
+                    const int i = order[(size_t)r * G + l];
+                    e.push_back(k < cnt(i) ? ent[ptr[i] + k] : 0u);
                 }
         }
         om = (int)T.size();
@@ -781,7 +783,9 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
         for (uint32_t h = m.Lptr[j]; h < m.Lptr[j + 1]; ++h)
             if ((m.Lent[h] >> 16) > 2) p.pfast = 0;
     }
-    rounds(m.R, m.Lptr, m.Lent, p.oPRm, p.oPRe, 1, p.pfast ? 2 : 0);
+    std::vector<int> rorder((size_t)((m.R + G - 1) / G) * G, -1);
+    for (int j = 0; j < m.R; ++j) rorder[j] = j;          // reaction j: lane j mod G (the selection's partials)
+    rounds(rorder, m.Lptr, m.Lent, p.oPRm, p.oPRe, 1, p.pfast ? 2 : 0);
     p.oDR = 0;
     if (p.pfast) {                        // dependent records, 16-byte aligned
         while (T.size() % 4) T.push_back(0u);
@@ -794,8 +798,19 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
             T.push_back(0u);
         }
     }
-    rounds(m.S, m.Vptr, m.Vent, p.oVRm, p.oVRe, 4, 0);
-    rounds(m.S, FSptr, FSent, p.oFRm, p.oFRe, 4, 0);
+    // species by entry count, longest first, so that a round's lanes carry
+    // lists of similar length (tau' is a minimum and the update is per
+    // species, so the species-to-lane map is free)
+    std::vector<int> sorder((size_t)((m.S + G - 1) / G) * G, -1);
+    for (int i = 0; i < m.S; ++i) sorder[i] = i;
+    std::stable_sort(sorder.begin(), sorder.begin() + m.S, [&](int a, int b) {
+        return std::max(m.Vptr[a + 1] - m.Vptr[a], FSptr[a + 1] - FSptr[a]) >
+               std::max(m.Vptr[b + 1] - m.Vptr[b], FSptr[b + 1] - FSptr[b]);
+    });
+    p.oSPI = (int)T.size();
+    for (int i : sorder) T.push_back(i < 0 ? 0xffffffffu : (uint32_t)i);
+    rounds(sorder, m.Vptr, m.Vent, p.oVRm, p.oVRe, 4, 0);
+    rounds(sorder, FSptr, FSent, p.oFRm, p.oFRe, 4, 0);
 }
 
 }  // namespace ctl

@@ -74,7 +74,9 @@ struct KParams {
     // var and fear round lengths are multiples of 4 (loads issue 4 at a time).
     // Reactant entries (species | multiplicity << 16), var entries of the
     // tau' sums and species-major fear entries (reaction | change << 16).
-    int oPRm, oPRe, oVRm, oVRe, oFRm, oFRe;
+    // Var and fear rounds list species in the order T[oSPI + r*G + l]
+    // (0xffffffff: none), longest lists first.
+    int oPRm, oPRe, oVRm, oVRe, oFRm, oFRe, oSPI;
     int oDR;                  // pfast: SSA dependent records {r, entry 0, entry 1, 0} (uint4, Dptr order)
     int pfast;                // every reaction has <= 2 reactant entries of multiplicity <= 2:
                               //   the reactant stream then has exactly 2 entries per round
