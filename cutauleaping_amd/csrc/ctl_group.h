@@ -72,6 +72,7 @@ enum : int {
     DPP_ROW_SHR8 = 0x118,
     DPP_ROW_MIRROR = 0x140,
     DPP_ROW_HALF_MIRROR = 0x141,
+    DPP_ROW_NEWBCAST15 = 0x15F,   // lane 15 of each row to the whole row
 };
 
 // Hillis-Steele inclusive scan over the group: v_l <- v_{l-d} + v_l for
@@ -80,7 +81,14 @@ enum : int {
 template <int G>
 __device__ __forceinline__ float gscan(float v, int l) {
     float t;
-    if (G <= 16) {
+    if (G == 16) {
+        // the group is the DPP row: a shifted-in lane reads 0, and 0 + v = v
+        // (v is a sum of non-negative terms started from +0, never -0)
+        v = dppf<DPP_ROW_SHR1>(v) + v;
+        v = dppf<DPP_ROW_SHR2>(v) + v;
+        v = dppf<DPP_ROW_SHR4>(v) + v;
+        v = dppf<DPP_ROW_SHR8>(v) + v;
+    } else if (G < 16) {
         if (G >= 2) { t = dppf<DPP_ROW_SHR1>(v); if (l >= 1) v = t + v; }
         if (G >= 4) { t = dppf<DPP_ROW_SHR2>(v); if (l >= 2) v = t + v; }
         if (G >= 8) { t = dppf<DPP_ROW_SHR4>(v); if (l >= 4) v = t + v; }
@@ -110,11 +118,7 @@ __device__ __forceinline__ float gbcast_last(float v, int l) {
         return l < 4 ? u : t;
     }
     const int wl = (int)(threadIdx.x & 63u);
-    if (G == 16) {
-        const float r0 = rdlane(v, 15), r1 = rdlane(v, 31), r2 = rdlane(v, 47), r3 = rdlane(v, 63);
-        const int q = wl >> 4;
-        return q == 0 ? r0 : (q == 1 ? r1 : (q == 2 ? r2 : r3));
-    }
+    if (G == 16) return dppf<DPP_ROW_NEWBCAST15>(v);
     if (G == 32) {
         const float r0 = rdlane(v, 31), r1 = rdlane(v, 63);
         return wl < 32 ? r0 : r1;
