@@ -175,14 +175,6 @@ struct GBatchParams {
     int32_t* X;                 // [slot][maxS]
 };
 
-__device__ __forceinline__ GView gview(uint32_t* smem, int S, int R) {
-    GView V;
-    V.Xb = (int32_t*)smem;
-    V.A = (uint32_t*)(V.Xb + 2 * S);
-    V.Kc = (int32_t*)(V.A + R);
-    V.Q = V.Kc + R;
-    return V;
-}
 
 __global__ __launch_bounds__(64) void gbatch_kernel(GBatchParams bp) {
     constexpr int G = 64;
@@ -204,7 +196,7 @@ __global__ __launch_bounds__(64) void gbatch_kernel(GBatchParams bp) {
     GView V;
     if (L.sim >= 0) {                    // resume a parked simulation
         p = desc_of_b(bp.desc, m);
-        V = gview(smem, p.S, p.R);
+        V = group_view(smem, p.S, p.R);
         L.t = bp.t[slot];
         L.cursor = bp.cursor[slot];
         L.evc = bp.evc[slot];
@@ -241,7 +233,7 @@ __global__ __launch_bounds__(64) void gbatch_kernel(GBatchParams bp) {
             m = __builtin_amdgcn_readfirstlane(CT_I32(bp.order)[lo]);
             const uint32_t r = (uint32_t)((int64_t)q - CT_I64(bp.cum)[lo]);
             p = desc_of_b(bp.desc, m);
-            V = gview(smem, p.S, p.R);
+            V = group_view(smem, p.S, p.R);
             L.sim = (int)r;
             L.t = 0.0;
             L.cursor = 0;
@@ -508,13 +500,13 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
     for (int i = 0; i < nmodels; ++i) {
         const ctl::Model& m = models[i]->m;
         lane_words = std::max(lane_words, (size_t)2 * m.S + m.R + (m.c_rows == 1 ? 0 : m.R));
-        gwords = std::max(gwords, (size_t)2 * m.S + (size_t)3 * m.R);
+        gwords = std::max(gwords, ctl::group_lds_words(m.S, m.R));
     }
     int G = o->lanes_per_sim;
     if (G == 0) G = lane_words * 4 * 64 <= 16384 ? 1 : 64;
     if (G != 1 && G != 64) return bail(ctl::fail(CTL_E_ARGS, "a batch runs with lanes_per_sim 0, 1 or 64"));
     if (G == 64 && gwords * 4 > 65536)
-        return bail(ctl::fail(CTL_E_LIMIT, "a model is too large for the group kernel (2S+3R > 16384)"));
+        return bail(ctl::fail(CTL_E_LIMIT, "a model is too large for the group kernel (more than 64 KB of LDS)"));
     b->lanes = G;
     int per_cu = 0;
     int rc;

@@ -7,7 +7,8 @@
 //     X   int32 [2][S]   state and trial state (parity bit swaps them)
 //     A   uint32[R]      propensity (FP32 bits) | critical flag << 31
 //     Kc  int32 [R]      counts drawn for the current leap attempt
-//     Q   int32 [R]      reactions queued for a Poisson draw (lambda >= 1)
+//     Q   uint16[R]      reactions queued for a Poisson draw (lambda >= 1), in
+//                        the trial-state half of X while it is free
 // so every table-driven access is an LDS access instead of a dependent HBM
 // load.  Group-wide values (a0, tau, the chosen reaction) are computed with
 // cross-lane shuffles and are identical in every lane of the group, so the
@@ -170,8 +171,25 @@ struct GView {
     int32_t* Xb;             // [2][S]
     uint32_t* A;             // [R]
     int32_t* Kc;             // [R]
-    int32_t* Q;              // [R]
+    uint16_t* Qx;            // [R] own queue region, or null: the queue lives in the trial state
 };
+
+// LDS words of one group's simulation: X[2][S], A[R], Kc[R], and the queue of
+// lambda >= 1 reactions as 16-bit indices, which shares the trial-state half
+// of X (free until the update writes it) when R/2 <= S, else follows Kc.
+__host__ __device__ __forceinline__ size_t group_lds_words(int S, int R) {
+    const size_t q = ((size_t)R + 1) / 2;
+    return (size_t)2 * S + (size_t)2 * R + (q > (size_t)S ? q : 0);
+}
+
+__device__ __forceinline__ GView group_view(uint32_t* base, int S, int R) {
+    GView V;
+    V.Xb = (int32_t*)base;
+    V.A = (uint32_t*)(V.Xb + 2 * S);
+    V.Kc = (int32_t*)(V.A + R);
+    V.Qx = ((size_t)R + 1) / 2 > (size_t)S ? (uint16_t*)(V.Kc + R) : nullptr;
+    return V;
+}
 
 // Propensity of reaction j from the group's LDS state; the arithmetic of
 // prop_one (CalculatePropensity, T.cu:1443-1470, in FP32).
@@ -426,6 +444,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         // parallel, each from the reaction's own stream (word 3 = 2^31 | j)
         rng_event(L.lg, e);
         uint64_t f = 0;
+        uint16_t* Q = V.Qx ? V.Qx : (uint16_t*)X1;
         double cum = 0.0, pos = (double)exp_rv(L.lg);
         L.gaps++;
         uint32_t nq = 0;
@@ -448,12 +467,12 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
                 f += (uint64_t)k;
             }
             const uint64_t m = gballot<G>(need);
-            if (need) V.Q[nq + (uint32_t)__popcll(m & ((1ull << l) - 1ull))] = j;
+            if (need) Q[nq + (uint32_t)__popcll(m & ((1ull << l) - 1ull))] = (uint16_t)j;
             nq += (uint32_t)__popcll(m);
         }
         gsync();
         for (uint32_t qi = (uint32_t)l; qi < nq; qi += G) {
-            const int j = V.Q[qi];
+            const int j = Q[qi];
             const float lam = tau * __uint_as_float(V.A[j]);
             Rng gr = L.gg;
             gr.tag = 0x80000000u | (uint32_t)j;
@@ -686,12 +705,7 @@ __global__ __launch_bounds__(256) void group_kernel(KParams p) {
     GLane L;
     L.slot = (int)blockIdx.x * (int)(blockDim.x / G) + gib;
     if (L.slot * G >= p.ns) return;
-    const size_t words = (size_t)2 * p.S + (size_t)3 * p.R;
-    GView V;
-    V.Xb = (int32_t*)(smem + (size_t)gib * words);
-    V.A = (uint32_t*)(V.Xb + 2 * p.S);
-    V.Kc = (int32_t*)(V.A + p.R);
-    V.Q = V.Kc + p.R;
+    const GView V = group_view(smem + (size_t)gib * group_lds_words(p.S, p.R), p.S, p.R);
     L.gg.k0 = L.lg.k0 = p.key0;
     L.gg.k1 = L.lg.k1 = p.key1;
     L.gg.tag = 0u;
