@@ -695,7 +695,7 @@ __device__ __forceinline__ void gfinish(const KParams& p, GLane& L, const GView&
 // with its SSA burst when it switched) per group, then parks the group's
 // state in the slot arrays for the next launch.
 template <int G>
-__global__ __launch_bounds__(256) void group_kernel(KParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 16 ? 4 : 1))) void group_kernel(KParams p) {
     extern __shared__ uint32_t smem[];
     __shared__ float recip[64];
     for (int i = threadIdx.x; i < 64; i += blockDim.x) recip[i] = c_recip[i];
