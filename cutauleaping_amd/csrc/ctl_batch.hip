@@ -27,6 +27,9 @@
 #include <numeric>
 #include <vector>
 
+// batch_kernel lanes read their own model's descriptor: parameters are not
+// known to be uniform
+#define CTL_UNIFORM_PARAMS 0
 #include "ctl_group.h"
 #include "ctl_kernels.h"
 

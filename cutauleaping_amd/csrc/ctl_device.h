@@ -16,6 +16,14 @@ namespace ctl {
 // 64-bit seed.  An "event" is one leap attempt, one tau-halving retry or one
 // SSA burst; its uniforms are consumed in order, four per Philox block.
 // ---------------------------------------------------------------------------
+// 1 when the kernels of the translation unit read their parameters (the
+// Philox key included) from kernel arguments, the same in every lane; a unit
+// whose lanes may read different parameter sets defines it 0 before
+// including this header.
+#ifndef CTL_UNIFORM_PARAMS
+#define CTL_UNIFORM_PARAMS 1
+#endif
+
 struct Rng {
     uint32_t k0, k1;     // key
     uint32_t sim;        // global simulation index
@@ -29,6 +37,13 @@ struct Rng {
 __device__ __forceinline__ void philox_block(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
                                              uint32_t k1, uint32_t& o0, uint32_t& o1, uint32_t& o2,
                                              uint32_t& o3) {
+#if CTL_UNIFORM_PARAMS
+    // The key is kernel-uniform: without this barrier the compiler hoists all
+    // ten round keys into scalar registers for the whole kernel, which pushes
+    // other uniform values into spill slots.  After it, the round keys are two
+    // scalar adds per round, issued beside the vector work.
+    asm volatile("" : "+s"(k0), "+s"(k1));
+#endif
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint32_t hi0 = __umulhi(0xD2511F53u, c0);

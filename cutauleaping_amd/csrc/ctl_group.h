@@ -47,6 +47,16 @@ __device__ __forceinline__ void gsync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A kernel parameter read at its point of use: the compiler may not hoist a
+// test of it out of the main loop, where it would hold a 64-bit lane mask in
+// scalar registers for the whole kernel (and spill others).
+__device__ __forceinline__ int kuse(int v) {
+#if CTL_UNIFORM_PARAMS
+    asm volatile("" : "+s"(v));
+#endif
+    return v;
+}
+
 template <int G>
 __device__ __forceinline__ uint64_t gballot(bool pred) {
     const uint64_t b = __ballot(pred);
@@ -245,7 +255,7 @@ __device__ __forceinline__ void gprops(const KParams& p, const int32_t* X, const
     const int nr = (p.R + G - 1) / G;
     P = 0.0f;
     PC = 0.0f;
-    if (p.pfast) {
+    if (kuse(p.pfast)) {
         // two entries per reaction; four rounds per batch of loads
         const uint32_t* E = p.T + p.oPRe + l;
         const uint32_t nc = (uint32_t)p.nc;
@@ -539,7 +549,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         }
     }
     L.xb ^= 1u;
-    if (p.nfs > 0) {                     // feeds (T.cu:2072-2079)
+    if (kuse(p.nfs) > 0) {               // feeds (T.cu:2072-2079)
         int32_t* Xn = V.Xb + (L.xb ? p.S : 0);
         for (int u = l; u < p.nfs; u += G) Xn[p.fs[u]] = (int32_t)L.fvrow[u];
         gsync();
@@ -600,14 +610,14 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
             X[s] = (int32_t)((uint32_t)X[s] + (uint32_t)v);
         }
         gsync();
-        if (p.nfs > 0) {                 // feeds (T.cu:2253-2258); their readers are in the dependency lists
+        if (kuse(p.nfs) > 0) {           // feeds (T.cu:2253-2258); their readers are in the dependency lists
             for (int u = l; u < p.nfs; u += G) X[p.fs[u]] = (int32_t)L.fvrow[u];
             gsync();
         }
         L.ssa_steps++;
         if (tn <= p.tmax) L.firings++;
         const uint32_t d0 = p.T[p.oDp + j], d1 = p.T[p.oDp + j + 1];
-        if (p.pfast) {
+        if (kuse(p.pfast)) {
             // dependent records {r, reactant entry 0, entry 1, 0}: one 16-byte load each
             const uint4* DR = reinterpret_cast<const uint4*>(p.T + p.oDR);
             const uint32_t nc = (uint32_t)p.nc;
@@ -735,7 +745,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 16 ? 4
     }
     for (int it = 0; it < p.iters; ++it) {
         if (L.sim < 0 && !gfetch<G>(p, L, V, l)) break;
-        if (p.force_ssa) {
+        if (kuse(p.force_ssa)) {
             if (L.t >= p.tmax || gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);
             continue;
         }
