@@ -150,6 +150,18 @@ __device__ __forceinline__ float gmin(float v) {
     return v;
 }
 
+// Maximum over a group of at most 16 lanes (inside one DPP row).
+template <int G>
+__device__ __forceinline__ int gmaxi(int v) {
+    static_assert(G <= 16, "DPP row reduction");
+    int o;
+    if (G >= 2) { o = __builtin_amdgcn_update_dpp(0, v, DPP_QP_1032, 0xf, 0xf, false); v = o > v ? o : v; }
+    if (G >= 4) { o = __builtin_amdgcn_update_dpp(0, v, DPP_QP_2301, 0xf, 0xf, false); v = o > v ? o : v; }
+    if (G >= 8) { o = __builtin_amdgcn_update_dpp(0, v, DPP_ROW_HALF_MIRROR, 0xf, 0xf, false); v = o > v ? o : v; }
+    if (G >= 16) { o = __builtin_amdgcn_update_dpp(0, v, DPP_ROW_MIRROR, 0xf, 0xf, false); v = o > v ? o : v; }
+    return v;
+}
+
 template <int G>
 __device__ __forceinline__ uint64_t gsum64(uint64_t v) {
 #pragma unroll
@@ -357,7 +369,8 @@ __device__ __forceinline__ int gselect(const KParams& p, const uint32_t* A, floa
         }
         if (pick < 0) pick = last;
     }
-    return __shfl(pick, sel, G);
+    if constexpr (G <= 16) return gmaxi<G>(pick);     // only lane sel holds a pick >= 0
+    else return __shfl(pick, sel, G);
 }
 
 template <int G>
