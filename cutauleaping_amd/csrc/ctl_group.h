@@ -713,6 +713,9 @@ __device__ __forceinline__ void gfinish(const KParams& p, GLane& L, const GView&
     L.sim = -1;
 }
 
+// Longest a switched group waits for a partner burst (leap iterations).
+constexpr int GBURST_WAIT = 48;
+
 // Persistent group kernel: p.ns lanes = p.ns / G group slots, each pulling
 // simulations from the queue; a launch runs up to p.iters leap attempts (each
 // with its SSA burst when it switched) per group, then parks the group's
@@ -756,19 +759,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 16 ? 4
         L.fvrow = p.nfs > 0 ? p.fv + (size_t)(L.sim / p.fv_div) * p.nfs : nullptr;
         gsync();
     }
+    // A group that switched to SSA holds its burst until a second group of the
+    // wavefront switches too (or it has waited GBURST_WAIT iterations, or every
+    // group still running is waiting): the bursts then run side by side instead
+    // of one group's burst idling the others.  Each simulation's trajectory is
+    // unchanged; only the order in which the wavefront serves its groups moves.
+    bool pend = false;
+    int waited = 0;
     for (int it = 0; it < p.iters; ++it) {
         if (L.sim < 0 && !gfetch<G>(p, L, V, l)) break;
         if (kuse(p.force_ssa)) {
             if (L.t >= p.tmax || gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);
             continue;
         }
-        const int r = gleap<G>(p, L, V, l, recip);
-        if (r == LEAP_DONE || r == LEAP_ABSORB) {
-            gfinish<G>(p, L, V, l);
-        } else if (r == LEAP_SWITCH) {
+        if (!pend) {
+            const int r = gleap<G>(p, L, V, l, recip);
+            if (r == LEAP_DONE || r == LEAP_ABSORB) {
+                gfinish<G>(p, L, V, l);
+            } else if (r == LEAP_SWITCH) {
+                pend = true;
+                waited = 0;
+            }
+        } else {
+            ++waited;
+        }
+        bool go = pend;
+        if (G < 64) {
+            const uint64_t lead = __ballot(l == 0);            // groups still in the loop
+            const uint64_t pm = __ballot(pend && l == 0);       // ... waiting for a burst
+            go = pm != 0 && (__popcll(pm) >= 2 || pm == lead || __ballot(pend && waited >= GBURST_WAIT) != 0);
+        }
+        if (go && pend) {
+            pend = false;
             if (gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);
         }
     }
+    if (pend && gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);   // before parking
     const int s0 = L.slot;
     if (L.sim >= 0) {                    // park: state to the slot arrays
         const int32_t* X = V.Xb + (L.xb ? p.S : 0);
