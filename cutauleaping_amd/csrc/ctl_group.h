@@ -714,7 +714,7 @@ __device__ __forceinline__ void gfinish(const KParams& p, GLane& L, const GView&
 }
 
 // Longest a switched group waits for a partner burst (leap iterations).
-constexpr int GBURST_WAIT = 48;
+constexpr int GBURST_WAIT = 24;
 
 // Persistent group kernel: p.ns lanes = p.ns / G group slots, each pulling
 // simulations from the queue; a launch runs up to p.iters leap attempts (each
