@@ -150,18 +150,6 @@ __device__ __forceinline__ float gmin(float v) {
     return v;
 }
 
-// Maximum over a group of at most 16 lanes (inside one DPP row).
-template <int G>
-__device__ __forceinline__ int gmaxi(int v) {
-    static_assert(G <= 16, "DPP row reduction");
-    int o;
-    if (G >= 2) { o = __builtin_amdgcn_update_dpp(0, v, DPP_QP_1032, 0xf, 0xf, false); v = o > v ? o : v; }
-    if (G >= 4) { o = __builtin_amdgcn_update_dpp(0, v, DPP_QP_2301, 0xf, 0xf, false); v = o > v ? o : v; }
-    if (G >= 8) { o = __builtin_amdgcn_update_dpp(0, v, DPP_ROW_HALF_MIRROR, 0xf, 0xf, false); v = o > v ? o : v; }
-    if (G >= 16) { o = __builtin_amdgcn_update_dpp(0, v, DPP_ROW_MIRROR, 0xf, 0xf, false); v = o > v ? o : v; }
-    return v;
-}
-
 template <int G>
 __device__ __forceinline__ uint64_t gsum64(uint64_t v) {
 #pragma unroll
@@ -345,6 +333,37 @@ __device__ __forceinline__ int gselect(const KParams& p, const uint32_t* A, floa
     const uint64_t hit = gballot<G>(Pown > 0.0f && P >= w);
     const int sel = hit ? __builtin_ctzll(hit) : 63 - __builtin_clzll(cand);
     const float excl = G <= 16 ? dppf<DPP_ROW_SHR1>(P) : __shfl_up(P, 1, G);
+    if constexpr (G <= 16) {
+        // The walk through lane sel's reactions sel + i*G, spread over the
+        // group: lane i holds the i-th propensity and the sequential sums
+        // excl + a_0 + ... + a_i are built by G-1 row shifts (after k shifts
+        // lanes 0..k hold their final sum; a shifted-in 0 or a skipped entry
+        // adds an exact 0), so every sum is rounded as in the one-lane walk.
+        const int nj = (p.R - sel + G - 1) / G;
+        float carry = __shfl(excl, sel, G);
+        if (sel == 0) carry = 0.0f;
+        int last = -1;
+        for (int c = 0; c < nj; c += G) {
+            const int i = c + l;
+            const int j = sel + i * G;
+            const uint32_t u = i < nj ? A[j] : 0u;
+            const bool use = which != 2 || (u & 0x80000000u) || j == 0;
+            const float a = use ? aval(u) : 0.0f;
+            const float a1 = l == 0 ? carry + a : a;
+            float x = a1;
+#pragma unroll
+            for (int k = 1; k < G; ++k) {
+                const float t = dppf<DPP_ROW_SHR1>(x) + a1;
+                x = (G == 16 || l > 0) ? t : a1;
+            }
+            const uint64_t pos = gballot<G>(i < nj && a > 0.0f);
+            const uint64_t hm = gballot<G>(i < nj && a > 0.0f && w <= x);
+            if (hm) return sel + (c + __builtin_ctzll(hm)) * G;
+            if (pos) last = sel + (c + 63 - __builtin_clzll(pos)) * G;
+            carry = gbcast_last<G>(x, l);
+        }
+        return last;
+    }
     int pick = -1;
     if (l == sel) {
         float acc = l > 0 ? excl : 0.0f;
@@ -369,8 +388,7 @@ __device__ __forceinline__ int gselect(const KParams& p, const uint32_t* A, floa
         }
         if (pick < 0) pick = last;
     }
-    if constexpr (G <= 16) return gmaxi<G>(pick);     // only lane sel holds a pick >= 0
-    else return __shfl(pick, sel, G);
+    return __shfl(pick, sel, G);
 }
 
 template <int G>
