@@ -429,7 +429,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
     const uint32_t* SPI = p.T + p.oSPI + l;
     for (int r = 0; r < nrs; ++r) {
         const uint32_t len = TC[p.oVRm + r];
-        const int sp = (int)SPI[r * G];
+        const uint32_t spw = SPI[r * G];             // species | HOR << 16 | HORtype << 24
         float mu = 0.0f, sg = 0.0f;
         for (uint32_t k = 0; k < len; k += 4) {      // len is a multiple of 4
             uint32_t e[4], u[4];
@@ -448,9 +448,12 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         }
         E += len * G;
         if (mu == 0.0f || sg == 0.0f) continue;      // no var entries (no species included)
-        const int32_t x = X[sp];
-        const float gg = gfun(p.T[p.oHw + sp], x);
-        const float tmp2 = (p.eps * (float)x) / gg;
+        const int32_t x = X[spw & 0xffffu];
+        const float ex = p.eps * (float)x;
+        // g is 1 or 2 unless some species has HOR 3 or HOR 2 of type 2: then
+        // the division by 2 is the exact scaling by 0.5
+        const float tmp2 = kuse(p.gsimple) ? (((spw >> 16) & 0xffu) == 2u ? ex * 0.5f : ex)
+                                           : ex / gfun(spw >> 16, x);
         const float macs = tmp2 > 1.0f ? tmp2 : 1.0f;
         const float t3 = macs / fabsf(mu);
         const float t4 = (macs * macs) / sg;
@@ -528,7 +531,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         const uint32_t* F = p.T + p.oFRe + l;
         for (int r = 0; r < nrs; ++r) {
             const uint32_t len = TC[p.oFRm + r];
-            const uint32_t sp = SPI[r * G];
+            const uint32_t sp = SPI[r * G] & 0xffffu;
             if (sp < (uint32_t)p.S) {
                 int64_t x = X[sp];
                 bool bs = false;
@@ -893,7 +896,12 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
                std::max(m.Vptr[b + 1] - m.Vptr[b], FSptr[b + 1] - FSptr[b]);
     });
     p.oSPI = (int)T.size();
-    for (int i : sorder) T.push_back(i < 0 ? 0xffffffffu : (uint32_t)i);
+    p.gsimple = 1;
+    for (int i = 0; i < m.S; ++i)
+        if (m.hor[i] >= 3 || (m.hor[i] == 2 && m.hort[i] == 2)) p.gsimple = 0;
+    for (int i : sorder)
+        T.push_back(i < 0 ? 0xffffffffu
+                          : (uint32_t)i | ((uint32_t)m.hor[i] << 16) | ((uint32_t)m.hort[i] << 24));
     rounds(sorder, m.Vptr, m.Vent, p.oVRm, p.oVRe, 4, 0);
     rounds(sorder, FSptr, FSent, p.oFRm, p.oFRe, 4, 0);
 }

@@ -75,8 +75,9 @@ struct KParams {
     // Reactant entries (species | multiplicity << 16), var entries of the
     // tau' sums and species-major fear entries (reaction | change << 16).
     // Var and fear rounds list species in the order T[oSPI + r*G + l]
-    // (0xffffffff: none), longest lists first.
-    int oPRm, oPRe, oVRm, oVRe, oFRm, oFRe, oSPI;
+    // (species | HOR << 16 | HORtype << 24; 0xffffffff: none), longest lists
+    // first.  gsimple: no species has HOR 3 or HOR 2 of type 2, so g is 1 or 2.
+    int oPRm, oPRe, oVRm, oVRe, oFRm, oFRe, oSPI, gsimple;
     int oDR;                  // pfast: SSA dependent records {r, entry 0, entry 1, 0} (uint4, Dptr order)
     int pfast;                // every reaction has <= 2 reactant entries of multiplicity <= 2:
                               //   the reactant stream then has exactly 2 entries per round
