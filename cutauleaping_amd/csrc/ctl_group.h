@@ -491,6 +491,13 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         uint16_t* Q = V.Qx ? V.Qx : (uint16_t*)X1;
         double cum = 0.0, pos = (double)exp_rv(L.lg);
         L.gaps++;
+        // The lane's second event position is drawn now, with the first, by
+        // every lane at once; drawn inside the loop below it would cost the
+        // wavefront a divergent draw for almost every reaction round.  It is
+        // the same value from the same stream (the lane stream serves only
+        // these gaps), counted as a gap when it is used.
+        double pos2 = pos + (double)exp_rv(L.lg);
+        bool have2 = true;
         uint32_t nq = 0;
         const int nit = (p.R + G - 1) / G;
         for (int i = 0; i < nit; ++i) {
@@ -506,7 +513,16 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
                 cum = small ? cum + (double)lam : cum;
                 // pos >= cum held before this reaction, so only a lane whose
                 // process mass grew draws gaps
-                while (pos < cum) { ++k; pos = pos + (double)exp_rv(L.lg); L.gaps++; }
+                while (pos < cum) {
+                    ++k;
+                    if (have2) {
+                        pos = pos2;
+                        have2 = false;
+                    } else {
+                        pos = pos + (double)exp_rv(L.lg);
+                    }
+                    L.gaps++;
+                }
                 V.Kc[j] = k;                         // a queued reaction's count is drawn below
                 f += (uint64_t)k;
             }
