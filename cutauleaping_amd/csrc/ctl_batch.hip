@@ -30,6 +30,7 @@
 // batch_kernel lanes read their own model's descriptor: parameters are not
 // known to be uniform
 #define CTL_UNIFORM_PARAMS 0
+#include "ctl_gbatch.h"
 #include "ctl_group.h"
 #include "ctl_kernels.h"
 
@@ -63,14 +64,6 @@ __device__ __forceinline__ KParams desc_of(const BatchParams& bp, int m) {
     return p;
 }
 
-__device__ __forceinline__ KParams desc_of_b(const KParams* desc, int m) {
-    KParams p;
-    const ctab_u32 src = CT_U32(reinterpret_cast<const uint32_t*>(desc + m));
-    uint32_t* dst = reinterpret_cast<uint32_t*>(&p);
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(KParams) / 4); ++i) dst[i] = src[i];
-    return p;
-}
 
 __global__ __launch_bounds__(256) void batch_kernel(BatchParams bp) {
     __shared__ float recip[64];
@@ -149,144 +142,6 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchParams bp) {
     if (L.sim >= 0) store_lane(desc_of(bp, m), L);
 }
 
-
-// ---------------------------------------------------------------------------
-// Group batch: one simulation per 64-lane wavefront (group algorithm, G = 64).
-struct GBatchParams {
-    const KParams* desc;        // per-model descriptors (device)
-    const int32_t* order;       // models in cost order
-    const int64_t* cum;         // cumulative replicas in that order [nmodels + 1]
-    int nmodels;
-    int64_t total;
-    uint32_t* next;             // global simulation counter
-    int ngroups;                // resident group slots (one wavefront each)
-    int iters;
-    int maxS;
-    // parked state per group slot
-    int32_t* slot_model;
-    int32_t* slot_sim;          // replica index, -1 none, -2 queue exhausted
-    double* t;
-    uint32_t* cursor;
-    uint32_t* evc;
-    uint64_t* firings;
-    uint64_t* ssa_steps;
-    uint32_t* leaps;
-    uint32_t* rejections;
-    uint32_t* switches;
-    uint64_t* poisson;          // [slot][64]
-    uint64_t* gaps;             // [slot][64]
-    int32_t* X;                 // [slot][maxS]
-};
-
-
-__global__ __launch_bounds__(64) void gbatch_kernel(GBatchParams bp) {
-    constexpr int G = 64;
-    extern __shared__ uint32_t smem[];
-    __shared__ float recip[64];
-    recip[threadIdx.x] = c_recip[threadIdx.x];
-    __syncthreads();
-    const int l = (int)threadIdx.x;
-    const int slot = (int)blockIdx.x;
-    if (slot >= bp.ngroups) return;
-    GLane L;
-    L.slot = slot;
-    L.gg.tag = 0u;
-    L.lg.tag = 1u + (uint32_t)l;
-    int m = __builtin_amdgcn_readfirstlane(bp.slot_model[slot]);
-    L.sim = __builtin_amdgcn_readfirstlane(bp.slot_sim[slot]);
-    if (L.sim == -2) return;
-    KParams p;
-    GView V;
-    if (L.sim >= 0) {                    // resume a parked simulation
-        p = desc_of_b(bp.desc, m);
-        V = group_view(smem, p.S, p.R);
-        L.t = bp.t[slot];
-        L.cursor = bp.cursor[slot];
-        L.evc = bp.evc[slot];
-        L.xb = 0;
-        L.firings = bp.firings[slot];
-        L.ssa_steps = bp.ssa_steps[slot];
-        L.leaps = bp.leaps[slot];
-        L.rejections = bp.rejections[slot];
-        L.switches = bp.switches[slot];
-        L.absorbed = 0;
-        L.poisson = bp.poisson[(size_t)slot * G + l];
-        L.gaps = bp.gaps[(size_t)slot * G + l];
-        L.gg.k0 = L.lg.k0 = p.key0;
-        L.gg.k1 = L.lg.k1 = p.key1;
-        L.gg.sim = L.lg.sim = p.sim_base + (uint32_t)L.sim;
-        L.gg.pos = L.lg.pos = 4;
-        for (int s = l; s < p.S; s += G) V.Xb[s] = bp.X[(size_t)slot * bp.maxS + s];
-        L.crate = grates(p, L.sim);
-        L.fvrow = p.nfs > 0 ? p.fv + (size_t)(L.sim / p.fv_div) * p.nfs : nullptr;
-        gsync();
-    }
-    for (int it = 0; it < bp.iters; ++it) {
-        if (L.sim < 0) {                 // next simulation of any model
-            uint32_t q = 0;
-            if (l == 0) q = atomicAdd(bp.next, 1u);
-            q = __builtin_amdgcn_readfirstlane(q);
-            if ((int64_t)q >= bp.total) { L.sim = -2; break; }
-            int lo = 0, hi = bp.nmodels - 1;           // last k with cum[k] <= q
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if ((int64_t)q >= CT_I64(bp.cum)[mid]) lo = mid; else hi = mid - 1;
-            }
-            lo = __builtin_amdgcn_readfirstlane(lo);
-            m = __builtin_amdgcn_readfirstlane(CT_I32(bp.order)[lo]);
-            const uint32_t r = (uint32_t)((int64_t)q - CT_I64(bp.cum)[lo]);
-            p = desc_of_b(bp.desc, m);
-            V = group_view(smem, p.S, p.R);
-            L.sim = (int)r;
-            L.t = 0.0;
-            L.cursor = 0;
-            L.evc = 0;
-            L.xb = 0;
-            L.firings = L.ssa_steps = 0;
-            L.leaps = L.rejections = L.switches = L.absorbed = 0;
-            L.poisson = L.gaps = 0;
-            L.gg.k0 = L.lg.k0 = p.key0;
-            L.gg.k1 = L.lg.k1 = p.key1;
-            L.gg.sim = L.lg.sim = p.sim_base + r;
-            L.gg.pos = L.lg.pos = 4;
-            for (int s = l; s < p.S; s += G)
-                V.Xb[s] = (int32_t)(p.x0_bcast ? p.x0_bcast[s] : p.x0_sims[(size_t)(r / (uint32_t)p.x0_div) * p.S + s]);
-            L.crate = grates(p, (int)r);
-            L.fvrow = p.nfs > 0 ? p.fv + (size_t)(r / (uint32_t)p.fv_div) * p.nfs : nullptr;
-            gsync();
-        }
-        if (p.force_ssa) {
-            if (L.t >= p.tmax || gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);
-            continue;
-        }
-        const int r = gleap<G>(p, L, V, l, recip);
-        if (r == LEAP_DONE || r == LEAP_ABSORB) {
-            gfinish<G>(p, L, V, l);
-        } else if (r == LEAP_SWITCH) {
-            if (gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);
-        }
-    }
-    if (L.sim >= 0) {                    // park
-        const int32_t* X = V.Xb + (L.xb ? p.S : 0);
-        for (int s = l; s < p.S; s += G) bp.X[(size_t)slot * bp.maxS + s] = X[s];
-        bp.poisson[(size_t)slot * G + l] = L.poisson;
-        bp.gaps[(size_t)slot * G + l] = L.gaps;
-        if (l == 0) {
-            bp.t[slot] = L.t;
-            bp.cursor[slot] = L.cursor;
-            bp.evc[slot] = L.evc;
-            bp.firings[slot] = L.firings;
-            bp.ssa_steps[slot] = L.ssa_steps;
-            bp.leaps[slot] = L.leaps;
-            bp.rejections[slot] = L.rejections;
-            bp.switches[slot] = L.switches;
-        }
-    }
-    if (l == 0) {
-        bp.slot_model[slot] = m;
-        bp.slot_sim[slot] = L.sim;
-    }
-}
 
 }  // namespace ctl
 

@@ -34,7 +34,7 @@ struct Rng {
     uint32_t tag = 0u;   // counter word 3: 0, or 1 + lane for a group's lane streams
 };
 
-__device__ __forceinline__ void philox_block(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+static __device__ __forceinline__ void philox_block(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
                                              uint32_t k1, uint32_t& o0, uint32_t& o1, uint32_t& o2,
                                              uint32_t& o3) {
 #if CTL_UNIFORM_PARAMS
@@ -42,6 +42,8 @@ __device__ __forceinline__ void philox_block(uint32_t c0, uint32_t c1, uint32_t 
     // ten round keys into scalar registers for the whole kernel, which pushes
     // other uniform values into spill slots.  After it, the round keys are two
     // scalar adds per round, issued beside the vector work.
+    k0 = __builtin_amdgcn_readfirstlane(k0);      // uniform; a no-op when already scalar
+    k1 = __builtin_amdgcn_readfirstlane(k1);
     asm volatile("" : "+s"(k0), "+s"(k1));
 #endif
 #pragma unroll

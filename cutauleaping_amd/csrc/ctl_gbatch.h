@@ -1,0 +1,49 @@
+// ctl_gbatch.h -- the group batch kernel's parameters and declaration (the
+// kernel is defined in ctl_gbatch.hip, the host runtime in ctl_batch.hip).
+#pragma once
+
+#include "ctl_kernels.h"
+
+namespace ctl {
+
+// Group batch: one simulation per 64-lane wavefront (group algorithm, G = 64).
+struct GBatchParams {
+    const KParams* desc;        // per-model descriptors (device)
+    const int32_t* order;       // models in cost order
+    const int64_t* cum;         // cumulative replicas in that order [nmodels + 1]
+    int nmodels;
+    int64_t total;
+    uint32_t* next;             // global simulation counter
+    int ngroups;                // resident group slots (one wavefront each)
+    int iters;
+    int maxS;
+    // parked state per group slot
+    int32_t* slot_model;
+    int32_t* slot_sim;          // replica index, -1 none, -2 queue exhausted
+    double* t;
+    uint32_t* cursor;
+    uint32_t* evc;
+    uint64_t* firings;
+    uint64_t* ssa_steps;
+    uint32_t* leaps;
+    uint32_t* rejections;
+    uint32_t* switches;
+    uint64_t* poisson;          // [slot][64]
+    uint64_t* gaps;             // [slot][64]
+    int32_t* X;                 // [slot][maxS]
+};
+
+// The descriptor of model m (wave-uniform), copied through a
+// constant-address-space pointer: a burst of scalar loads.
+static __device__ __forceinline__ KParams desc_of_b(const KParams* desc, int m) {
+    KParams p;
+    const ctab_u32 src = CT_U32(reinterpret_cast<const uint32_t*>(desc + m));
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&p);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(KParams) / 4); ++i) dst[i] = src[i];
+    return p;
+}
+
+__global__ void gbatch_kernel(GBatchParams bp);
+
+}  // namespace ctl

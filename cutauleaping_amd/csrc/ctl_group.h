@@ -50,8 +50,9 @@ __device__ __forceinline__ void gsync() {
 // A kernel parameter read at its point of use: the compiler may not hoist a
 // test of it out of the main loop, where it would hold a 64-bit lane mask in
 // scalar registers for the whole kernel (and spill others).
-__device__ __forceinline__ int kuse(int v) {
+static __device__ __forceinline__ int kuse(int v) {
 #if CTL_UNIFORM_PARAMS
+    v = __builtin_amdgcn_readfirstlane(v);         // uniform; a no-op when already scalar
     asm volatile("" : "+s"(v));
 #endif
     return v;
