@@ -17,8 +17,9 @@
 // Algorithm (restated lane by lane by the oracle's group mode):
 //  * propensities and criticality per reaction as in the one-lane kernel;
 //  * a0 and every cumulative selection use the lane partial sums (each
-//    lane's reactions in ascending order) combined by a Hillis-Steele
-//    inclusive scan over the G lanes; a selection takes the first lane with
+//    lane's reactions in ascending order) combined by an inclusive scan over
+//    the G lanes (Hillis-Steele inside rows of 16 lanes, then the row totals
+//    carried across rows; see gscan); a selection takes the first lane with
 //    a positive partial whose inclusive sum reaches the threshold (else the
 //    last lane with a positive partial), then walks that lane's reactions;
 //  * tau' per species as in the one-lane kernel, minimum across lanes;
@@ -85,11 +86,16 @@ enum : int {
     DPP_ROW_MIRROR = 0x140,
     DPP_ROW_HALF_MIRROR = 0x141,
     DPP_ROW_NEWBCAST15 = 0x15F,   // lane 15 of each row to the whole row
+    DPP_WAVE_SHR1 = 0x138,        // lane l reads lane l - 1 across rows
+    DPP_ROW_BCAST15 = 0x142,      // lane 15 of each row to the next row
+    DPP_ROW_BCAST31 = 0x143,      // lane 31 to rows 2 and 3
 };
 
-// Hillis-Steele inclusive scan over the group: v_l <- v_{l-d} + v_l for
-// d = 1, 2, 4, ...  A group of at most 16 lanes lies inside one DPP row, so
-// every step is a row shift; wider groups span rows and use shuffles.
+// Inclusive scan over the group.  Inside each DPP row of 16 lanes (the whole
+// group when G <= 16) a Hillis-Steele scan, v_l <- v_{l-d} + v_l for
+// d = 1, 2, 4, 8; wider groups then add the previous row's total to rows 1
+// and 3 (row_bcast:15) and, for G = 64, the first two rows' total to rows 2
+// and 3 (row_bcast:31).
 template <int G>
 __device__ __forceinline__ float gscan(float v, int l) {
     float t;
@@ -106,11 +112,13 @@ __device__ __forceinline__ float gscan(float v, int l) {
         if (G >= 8) { t = dppf<DPP_ROW_SHR4>(v); if (l >= 4) v = t + v; }
         if (G >= 16) { t = dppf<DPP_ROW_SHR8>(v); if (l >= 8) v = t + v; }
     } else {
-#pragma unroll
-        for (int d = 1; d < G; d <<= 1) {
-            t = __shfl_up(v, d, G);
-            if (l >= d) v = t + v;
-        }
+        v = dppf<DPP_ROW_SHR1>(v) + v;              // a shifted-in lane reads 0
+        v = dppf<DPP_ROW_SHR2>(v) + v;
+        v = dppf<DPP_ROW_SHR4>(v) + v;
+        v = dppf<DPP_ROW_SHR8>(v) + v;
+        v = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), DPP_ROW_BCAST15, 0xa, 0xf, false)) + v;
+        if (G == 64)
+            v = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), DPP_ROW_BCAST31, 0xc, 0xf, false)) + v;
     }
     return v;
 }
@@ -146,8 +154,12 @@ __device__ __forceinline__ float gmin(float v) {
     if (G >= 4) { o = dppf<DPP_QP_2301>(v); v = o < v ? o : v; }
     if (G >= 8) { o = dppf<DPP_ROW_HALF_MIRROR>(v); v = o < v ? o : v; }
     if (G >= 16) { o = dppf<DPP_ROW_MIRROR>(v); v = o < v ? o : v; }
-    if (G >= 32) { o = __shfl_xor(v, 16, G); v = o < v ? o : v; }
-    if (G >= 64) { o = __shfl_xor(v, 32, G); v = o < v ? o : v; }
+    if (G >= 32) {                                  // combine the rows' minima
+        const float r0 = rdlane(v, 15), r1 = rdlane(v, 31), r2 = rdlane(v, 47), r3 = rdlane(v, 63);
+        const float m01 = r1 < r0 ? r1 : r0, m23 = r3 < r2 ? r3 : r2;
+        if (G == 64) v = m23 < m01 ? m23 : m01;
+        else v = (threadIdx.x & 32u) ? m23 : m01;
+    }
     return v;
 }
 
@@ -333,7 +345,7 @@ __device__ __forceinline__ int gselect(const KParams& p, const uint32_t* A, floa
     if (cand == 0) return -1;
     const uint64_t hit = gballot<G>(Pown > 0.0f && P >= w);
     const int sel = hit ? __builtin_ctzll(hit) : 63 - __builtin_clzll(cand);
-    const float excl = G <= 16 ? dppf<DPP_ROW_SHR1>(P) : __shfl_up(P, 1, G);
+    const float excl = G <= 16 ? dppf<DPP_ROW_SHR1>(P) : dppf<DPP_WAVE_SHR1>(P);   // lane 0's value is unused
     if constexpr (G <= 16) {
         // The walk through lane sel's reactions sel + i*G, spread over the
         // group: lane i holds the i-th propensity and the sequential sums

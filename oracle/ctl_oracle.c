@@ -809,11 +809,26 @@ typedef struct {
     float* O;            /* G own partials (before the scan) */
 } grpctx;
 
-/* Hillis-Steele inclusive scan over G lanes: v[l] += v[l - d], d = 1, 2, 4... */
+/* Inclusive scan over the G lanes of a group, in the kernels' order: a
+ * Hillis-Steele scan (v[l] += v[l - d], d = 1, 2, 4, 8) inside rows of 16
+ * lanes, then the row totals carried across rows (G = 32, 64). */
 static void g_scan(float* v, float* tmp, int G) {
-    for (int d = 1; d < G; d <<= 1) {
+    /* Hillis-Steele inside rows of 16 lanes (the whole group when G <= 16) */
+    const int W = G < 16 ? G : 16;
+    for (int d = 1; d < W; d <<= 1) {
         memcpy(tmp, v, sizeof(float) * (size_t)G);
-        for (int l = d; l < G; ++l) v[l] = tmp[l - d] + tmp[l];
+        for (int l = 0; l < G; ++l)
+            if (l % W >= d) v[l] = tmp[l - d] + tmp[l];
+    }
+    if (G >= 32) {            /* rows 1 and 3 add the previous row's last lane */
+        memcpy(tmp, v, sizeof(float) * (size_t)G);
+        for (int l = 16; l < 32; ++l) v[l] = tmp[15] + tmp[l];
+        if (G == 64)
+            for (int l = 48; l < 64; ++l) v[l] = tmp[47] + tmp[l];
+    }
+    if (G == 64) {            /* rows 2 and 3 add lane 31 (the first two rows' total) */
+        memcpy(tmp, v, sizeof(float) * (size_t)G);
+        for (int l = 32; l < 64; ++l) v[l] = tmp[31] + tmp[l];
     }
 }
 
