@@ -41,7 +41,8 @@ def signature(left, right, x0, c, ts, feed=None) -> str:
 
 
 def _fits_group(S: int, R: int) -> bool:
-    return (2 * S + 2 * R + max(0, (R + 1) // 2 - S)) * 4 <= 65536
+    Rp = (R + 64) // 64 * 64          # the padded reaction arrays at the widest group
+    return (2 * S + 2 * Rp + max(0, (R + 1) // 2 - S)) * 4 <= 65536
 
 
 def tune(left, right, x0, c, ts, *, feed=None, cs=None, gpu=0, probe_sims=65536, probe_fraction=0.05,

@@ -358,7 +358,7 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
     for (int i = 0; i < nmodels; ++i) {
         const ctl::Model& m = models[i]->m;
         lane_words = std::max(lane_words, (size_t)2 * m.S + m.R + (m.c_rows == 1 ? 0 : m.R));
-        gwords = std::max(gwords, ctl::group_lds_words(m.S, m.R));
+        gwords = std::max(gwords, ctl::group_lds_words(m.S, m.R, 64));
     }
     int G = o->lanes_per_sim;
     if (G == 0) G = lane_words * 4 * 64 <= 16384 ? 1 : 64;

@@ -34,7 +34,8 @@ __global__ __launch_bounds__(64) void gbatch_kernel(GBatchParams bp) {
     GView V;
     if (L.sim >= 0) {                    // resume a parked simulation
         p = desc_of_b(bp.desc, m);
-        V = group_view(smem, p.S, p.R);
+        V = group_view(smem, p.S, p.R, G);
+        gzero_pad<G>(V, p.R, l);
         L.t = bp.t[slot];
         L.cursor = bp.cursor[slot];
         L.evc = bp.evc[slot];
@@ -71,7 +72,8 @@ __global__ __launch_bounds__(64) void gbatch_kernel(GBatchParams bp) {
             m = __builtin_amdgcn_readfirstlane(CT_I32(bp.order)[lo]);
             const uint32_t r = (uint32_t)((int64_t)q - CT_I64(bp.cum)[lo]);
             p = desc_of_b(bp.desc, m);
-            V = group_view(smem, p.S, p.R);
+            V = group_view(smem, p.S, p.R, G);
+            gzero_pad<G>(V, p.R, l);
             L.sim = (int)r;
             L.t = 0.0;
             L.cursor = 0;

@@ -192,26 +192,39 @@ struct GLane {
 
 struct GView {
     int32_t* Xb;             // [2][S]
-    uint32_t* A;             // [R]
-    int32_t* Kc;             // [R]
+    uint32_t* A;             // [Rp] (entries R .. Rp-1 stay 0)
+    int32_t* Kc;             // [Rp]
     uint16_t* Qx;            // [R] own queue region, or null: the queue lives in the trial state
 };
 
-// LDS words of one group's simulation: X[2][S], A[R], Kc[R], and the queue of
-// lambda >= 1 reactions as 16-bit indices, which shares the trial-state half
-// of X (free until the update writes it) when R/2 <= S, else follows Kc.
-__host__ __device__ __forceinline__ size_t group_lds_words(int S, int R) {
+// Padded reaction count of the LDS arrays: R + 1 rounded up to a multiple of
+// G.  The entries past R stay zero in A, so a loop over a lane's rounds needs
+// no per-lane bounds test and A[R] is a zero propensity that padding entries
+// of the tau' stream point at.
+__host__ __device__ __forceinline__ int group_rpad(int R, int G) { return (R + G) / G * G; }
+
+// LDS words of one group's simulation: X[2][S], A[Rp], Kc[Rp], and the queue
+// of lambda >= 1 reactions as 16-bit indices, which shares the trial-state
+// half of X (free until the update writes it) when R/2 <= S, else follows Kc.
+__host__ __device__ __forceinline__ size_t group_lds_words(int S, int R, int G) {
     const size_t q = ((size_t)R + 1) / 2;
-    return (size_t)2 * S + (size_t)2 * R + (q > (size_t)S ? q : 0);
+    return (size_t)2 * S + (size_t)2 * group_rpad(R, G) + (q > (size_t)S ? q : 0);
 }
 
-__device__ __forceinline__ GView group_view(uint32_t* base, int S, int R) {
+__device__ __forceinline__ GView group_view(uint32_t* base, int S, int R, int G) {
     GView V;
+    const int Rp = group_rpad(R, G);
     V.Xb = (int32_t*)base;
     V.A = (uint32_t*)(V.Xb + 2 * S);
-    V.Kc = (int32_t*)(V.A + R);
-    V.Qx = ((size_t)R + 1) / 2 > (size_t)S ? (uint16_t*)(V.Kc + R) : nullptr;
+    V.Kc = (int32_t*)(V.A + Rp);
+    V.Qx = ((size_t)R + 1) / 2 > (size_t)S ? (uint16_t*)(V.Kc + Rp) : nullptr;
     return V;
+}
+
+// Zeroes a group's A padding (R .. Rp-1); done when the group's model is set.
+template <int G>
+__device__ __forceinline__ void gzero_pad(const GView& V, int R, int l) {
+    for (int j = R + l; j < group_rpad(R, G); j += G) V.A[j] = 0u;
 }
 
 // Propensity of reaction j from the group's LDS state; the arithmetic of
@@ -453,8 +466,8 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int v = (int)(int16_t)(e[i] >> 16);
-                // critical reactions and padding (v = 0) add an exact zero
-                const float a = ((u[i] & 0x80000000u) || v == 0) ? 0.0f : __uint_as_float(u[i]);
+                // critical reactions and padding (A[R] = 0) add an exact zero
+                const float a = (u[i] & 0x80000000u) ? 0.0f : __uint_as_float(u[i]);
                 mu = mu + (float)v * a;
                 sg = sg + (float)(v * v) * a;
             }
@@ -638,10 +651,11 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
     gsync();
     for (int step = 0; step < p.ssa_steps; ++step) {
         float P = 0.0f;
-        for (int j0 = l; j0 < p.R; j0 += 4 * G) {      // past the end adds an exact 0
+        const int nrp = group_rpad(p.R, G) / G;        // rounds over A with its zero padding
+        for (int r0 = 0; r0 < nrp; r0 += 4) {          // padding adds an exact 0
             uint32_t u[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) u[i] = j0 + i * G < p.R ? V.A[j0 + i * G] : 0u;
+            for (int i = 0; i < 4; ++i) u[i] = r0 + i < nrp ? V.A[l + (r0 + i) * G] : 0u;
 #pragma unroll
             for (int i = 0; i < 4; ++i) P = P + aval(u[i]);
         }
@@ -781,7 +795,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 16 ? 4
     GLane L;
     L.slot = (int)blockIdx.x * (int)(blockDim.x / G) + gib;
     if (L.slot * G >= p.ns) return;
-    const GView V = group_view(smem + (size_t)gib * group_lds_words(p.S, p.R), p.S, p.R);
+    const GView V = group_view(smem + (size_t)gib * group_lds_words(p.S, p.R, G), p.S, p.R, G);
+    gzero_pad<G>(V, p.R, l);
     L.gg.k0 = L.lg.k0 = p.key0;
     L.gg.k1 = L.lg.k1 = p.key1;
     L.gg.tag = 0u;
@@ -874,7 +889,8 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
     // every round has exactly that many entries per lane; else the round's
     // longest list rounded up to a multiple of quantum.
     auto rounds = [&](const std::vector<int>& order, const std::vector<uint32_t>& ptr,
-                      const std::vector<uint32_t>& ent, int& om, int& oe, uint32_t quantum, uint32_t fixed) {
+                      const std::vector<uint32_t>& ent, int& om, int& oe, uint32_t quantum, uint32_t fixed,
+                      uint32_t pad) {
         const int nr = (int)(order.size() / (size_t)G);
         auto cnt = [&](int i) { return i < 0 ? 0u : ptr[i + 1] - ptr[i]; };
         std::vector<uint32_t> len((size_t)nr, 0u), e;
@@ -886,7 +902,7 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
             for (uint32_t k = 0; k < mx; ++k)
                 for (int l = 0; l < G; ++l) {
                     const int i = order[(size_t)r * G + l];
-                    e.push_back(k < cnt(i) ? ent[ptr[i] + k] : 0u);
+                    e.push_back(k < cnt(i) ? ent[ptr[i] + k] : pad);
                 }
         }
         om = (int)T.size();
@@ -902,7 +918,7 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
     }
     std::vector<int> rorder((size_t)((m.R + G - 1) / G) * G, -1);
     for (int j = 0; j < m.R; ++j) rorder[j] = j;          // reaction j: lane j mod G (the selection's partials)
-    rounds(rorder, m.Lptr, m.Lent, p.oPRm, p.oPRe, 1, p.pfast ? 2 : 0);
+    rounds(rorder, m.Lptr, m.Lent, p.oPRm, p.oPRe, 1, p.pfast ? 2 : 0, 0u);
     p.oDR = 0;
     if (p.pfast) {                        // dependent records, 16-byte aligned
         while (T.size() % 4) T.push_back(0u);
@@ -931,8 +947,8 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
     for (int i : sorder)
         T.push_back(i < 0 ? 0xffffffffu
                           : (uint32_t)i | ((uint32_t)m.hor[i] << 16) | ((uint32_t)m.hort[i] << 24));
-    rounds(sorder, m.Vptr, m.Vent, p.oVRm, p.oVRe, 4, 0);
-    rounds(sorder, FSptr, FSent, p.oFRm, p.oFRe, 4, 0);
+    rounds(sorder, m.Vptr, m.Vent, p.oVRm, p.oVRe, 4, 0, (uint32_t)m.R);   // padding: the zero slot A[R]
+    rounds(sorder, FSptr, FSent, p.oFRm, p.oFRe, 4, 0, 0u);
 }
 
 }  // namespace ctl

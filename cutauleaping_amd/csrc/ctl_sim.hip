@@ -320,10 +320,10 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     // per simulation with the simulation's state in LDS (group kernel).
     const size_t lane_words = (size_t)2 * S + R + (m.c_rows == 1 ? 0 : R);
     const bool lds_auto = lane_words * 4 * 64 <= 16384;
-    const size_t gwords = group_lds_words(S, R);           // group LDS words per simulation
+    const size_t gwords64 = group_lds_words(S, R, 64);     // largest group LDS words per simulation
     int G = o->lanes_per_sim;
     if (G == 0) {
-        if (lds_auto || gwords * 4 > 65536) {
+        if (lds_auto || gwords64 * 4 > 65536) {
             G = 1;
         } else {
             G = 4;
@@ -335,6 +335,7 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     s->lanes = G;
     int per_cu = 0;
     if (G > 1) {
+        const size_t gwords = group_lds_words(S, R, G);    // group LDS words per simulation
         if (gwords * 4 > 65536) return bail(ctl::fail(CTL_E_LIMIT, "model too large for the group kernel (more than 64 KB of LDS)"));
         int gpb = 256 / G;                                    // groups per workgroup
         while (gpb > 64 / G && (size_t)gpb * gwords * 4 > 65536) gpb /= 2;
