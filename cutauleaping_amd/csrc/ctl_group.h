@@ -163,8 +163,23 @@ __device__ __forceinline__ float gmin(float v) {
     return v;
 }
 
+template <int CTRL>
+__device__ __forceinline__ uint64_t dppu64(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xf, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xf, 0xf, false);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 template <int G>
 __device__ __forceinline__ uint64_t gsum64(uint64_t v) {
+    if constexpr (G <= 16) {
+        // butterfly inside the DPP row (the pairings of gmin); integer sums are exact in any order
+        if (G >= 2) v += dppu64<DPP_QP_1032>(v);
+        if (G >= 4) v += dppu64<DPP_QP_2301>(v);
+        if (G >= 8) v += dppu64<DPP_ROW_HALF_MIRROR>(v);
+        if (G >= 16) v += dppu64<DPP_ROW_MIRROR>(v);
+        return v;
+    }
 #pragma unroll
     for (int m = G / 2; m >= 1; m >>= 1) {
         const uint32_t lo = __shfl_xor((uint32_t)v, m, G);
@@ -308,7 +323,7 @@ __device__ __forceinline__ void gprops(const KParams& p, const int32_t* X, const
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int r = r0 + q, j = l + r * G;
-                if (r < nr && j < p.R) {
+                if (r < nr) {                          // past R: 0 (rate 0) into A's zero padding
                     float a = cr[q];
                     bool c = false;
                     const int32_t xs[2] = {x0[q], x1[q]};
@@ -529,7 +544,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         for (int i = 0; i < nit; ++i) {
             const int j = l + i * G;
             bool need = false;
-            if (j < p.R) {
+            {                                          // past R: the zero padding of A and Kc
                 const uint32_t u = V.A[j];
                 const bool crit = (u & 0x80000000u) != 0u;
                 const float lam = tau * __uint_as_float(u);
