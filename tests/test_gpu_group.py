@@ -33,11 +33,13 @@ def test_group_schlogl_bit_exact(gpu, oracle_lib, G, force):
     (14, 9, 9, 257, False, 3, 2, {"slots": 24, "iters_per_launch": 9}),
     (15, 30, 70, 129, True, 5, 32, {"iters_per_launch": 17}),
     (16, 40, 90, 77, False, 0, 64, {}),
+    (17, 25, 60, 200, True, 11, 16, {"iters_per_launch": 7}),
 ])
 def test_group_random_networks_bit_exact(gpu, oracle_lib, seed, N, M, n, per_sim, offset, G, geom):
     """Per-simulation rates and initial states, feed locks, sampled-species
     subsets, ragged sizes, every group width, input offsets, and small
-    iteration budgets / slot counts that park simulations across launches."""
+    iteration budgets / slot counts that park simulations across launches
+    (with SSA bursts held for a partner group flushed before parking)."""
     print(f"case: group random seed={seed} N={N} M={M} n={n} G={G} {geom}", flush=True)
     net, feed, x0, c, cs, ts = _random_case(seed, N, M, n, per_sim)
     m = gpu.Model.from_arrays(net["left"], net["right"], x0, c, ts, feed=feed, cs=cs)
