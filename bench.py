@@ -42,7 +42,7 @@ VALU_PEAK = 256 * 64 * 2.4e9          # lane-ops/s (MI355X: 256 CUs, 64 lanes/cl
 HBM_PEAK = 8.0e12                     # bytes/s
 # HBM bytes per simulation of this workload measured with rocprofv3 FETCH_SIZE / WRITE_SIZE,
 # scaled by a 1 GiB copy (profiles/group_kernel_c1_hbm_traffic.md)
-MEASURED_BYTES_PER_SIM = 84169
+MEASURED_BYTES_PER_SIM = 84023
 
 
 def lane_ops(stats, S, R, nL, nV, nF, SV):
