@@ -799,12 +799,12 @@ constexpr int GBURST_WAIT = 24;
 // simulations from the queue; a launch runs up to p.iters leap attempts (each
 // with its SSA burst when it switched) per group, then parks the group's
 // state in the slot arrays for the next launch.
-// G = 8 and 16 are compiled for four waves per SIMD (128 VGPRs, a few bytes
-// of scratch): measured +9 % at G = 16 on the 100x200 network and +24 % at
-// G = 8 on the 50x100 sweep network.  G = 4 groups are bounded by LDS
-// instead, and G = 32, 64 would spill heavily.
+// G = 8, 16 and 32 are compiled for four waves per SIMD (128 VGPRs, a few
+// bytes of scratch): measured +9 % at G = 16 and +4 % at G = 32 on the
+// 100x200 network, +24 % at G = 8 on the 50x100 sweep network.  G = 4
+// groups are bounded by LDS instead, and G = 64 would spill heavily.
 template <int G>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G == 16 ? 4 : 1))) void group_kernel(KParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G == 16 || G == 32 ? 4 : 1))) void group_kernel(KParams p) {
     extern __shared__ uint32_t smem[];
     __shared__ float recip[64];
     for (int i = threadIdx.x; i < 64; i += blockDim.x) recip[i] = c_recip[i];
