@@ -2,17 +2,19 @@
 // power of two) cooperate on one simulation, for networks whose per-lane
 // working set does not fit on chip.  Included by ctl_sim.hip.
 //
-// Work split: reaction j and species s belong to lane j mod G and s mod G.
-// A group's simulation state lives in LDS for the whole launch:
+// Work split: reaction j belongs to lane j mod G; species go to lanes in
+// rounds of G, longest var/fear lists first (the round streams).  A group's
+// simulation state lives in LDS for the whole launch:
 //     X   int32 [2][S]   state and trial state (parity bit swaps them)
-//     A   uint32[R]      propensity (FP32 bits) | critical flag << 31
-//     Kc  int32 [R]      counts drawn for the current leap attempt
+//     A   uint32[Rp]     propensity (FP32 bits) | critical flag << 31; zero
+//                        past R (Rp = R + 1 rounded up to a multiple of G)
+//     Kc  int32 [Rp]     counts drawn for the current leap attempt
 //     Q   uint16[R]      reactions queued for a Poisson draw (lambda >= 1), in
 //                        the trial-state half of X while it is free
 // so every table-driven access is an LDS access instead of a dependent HBM
 // load.  Group-wide values (a0, tau, the chosen reaction) are computed with
-// cross-lane shuffles and are identical in every lane of the group, so the
-// group's control flow is uniform.
+// DPP row operations and lane reads and are identical in every lane of the
+// group, so the group's control flow is uniform.
 //
 // Algorithm (restated lane by lane by the oracle's group mode):
 //  * propensities and criticality per reaction as in the one-lane kernel;
