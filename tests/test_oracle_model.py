@@ -33,7 +33,7 @@ def test_schlogl_compile(oracle_lib):
     got = [(r, s, v) for r in range(4) for s, v in zip(m["Ls"][m["Lptr"][r]:m["Lptr"][r + 1]],
                                                          m["Lv"][m["Lptr"][r]:m["Lptr"][r + 1]])]
     assert got == [(0, 0, 1), (0, 1, 2), (1, 1, 3), (2, 2, 1), (3, 1, 1)]
-    # rav: species-major net change, feed-locked species included (quirk Q1)
+    # rav: species-major net change, feed-locked species included (as the reference builds it)
     rav = [(r, s, v) for s in range(3) for r, v in zip(m["Vr"][m["Vptr"][s]:m["Vptr"][s + 1]],
                                                          m["Vv"][m["Vptr"][s]:m["Vptr"][s + 1]])]
     assert rav == [(0, 0, -1), (1, 0, 1), (0, 1, 1), (1, 1, -1), (2, 1, 1), (3, 1, -1), (2, 2, -1), (3, 2, 1)]

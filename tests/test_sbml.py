@@ -1,4 +1,4 @@
-"""SBML -> input-folder converter (SURVEY 8(f) #1) on the CPU."""
+"""SBML -> input-folder converter on the CPU."""
 import os
 
 import numpy as np
