@@ -64,29 +64,29 @@ def lane_ops(stats, S, R, nL, nV, nF, SV):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--sims", type=int, default=1 << 20, help="simulations in the workload")
-    ap.add_argument("--batch", type=int, default=1 << 18, help="simulations per step")
+    ap.add_argument("--slice", type=int, default=1 << 16, help="simulations per step")
     ap.add_argument("--t-end", type=float, default=50.0)
     ap.add_argument("--samples", type=int, default=200)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-sims", type=int, default=128)
+    ap.add_argument("--cpu-sims", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--iters", type=int, default=0)
     ap.add_argument("--ssa-chunk", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per simulation (0: library's choice)")
-    ap.add_argument("--workload", default="c1", choices=("c1", "c2", "c3", "c4"),
-                    help="c1: the headline (default); c2: batched uneven models; c3: surrogate-screened "
-                         "sweep; c4: enzyme cascade (adaptive and force-SSA)")
-    ap.add_argument("--models", type=int, default=1024, help="c2: models in the batch")
-    ap.add_argument("--c3-train", type=int, default=8192)
+    ap.add_argument("--workload", default="c1", choices=("c0", "c1", "c2", "c3", "c4"),
+                    help="c1: the headline (default); c0: Schlogl; c2: batched uneven models; "
+                         "c3: surrogate-screened sweep; c4: enzyme cascade (adaptive and force-SSA)")
+    ap.add_argument("--models", type=int, default=8192, help="c2: models in the batch")
+    ap.add_argument("--c3-train", type=int, default=65536)
     ap.add_argument("--c3-test", type=int, default=4096)
-    ap.add_argument("--c3-replicas", type=int, default=128)
+    ap.add_argument("--c3-replicas", type=int, default=256)
     ap.add_argument("--c3-judge", type=int, default=512)
-    ap.add_argument("--c3-epochs", type=int, default=200)
+    ap.add_argument("--c3-epochs", type=int, default=60)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device once (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -98,14 +98,14 @@ def main():
               "multi-GPU sharding by input offset is documented, not launched).", file=sys.stderr)
         sys.exit(2)
     if args.workload != "c1":
-        {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[args.workload](args)
+        {"c0": bench_c0, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[args.workload](args)
         return
 
     import cutauleaping_amd as ctl
     from cutauleaping_amd import networks
 
     if not os.path.exists(ctl.LIB_PATH):
-        ctl.build()
+        raise SystemExit("bench.py: libctl.so is not built (python __graft_entry__.py build)")
     t0 = time.time()
     net = networks.mass_action(100, 200, seed=args.seed, t_end=args.t_end)
     ts = networks.sample_times(args.samples, args.t_end)
@@ -116,40 +116,50 @@ def main():
     st = model.structure()
     SV = int(len(np.unique(st["rav_flat"][:, 1]))) if info.nV else 0
     threads = 256
-    batch = min(args.batch, args.sims)
-    blocks = (batch + threads - 1) // threads
-    n = threads * blocks
-    nbatches = max(1, args.sims // n)
-    sims = [ctl.Sim(model, threads, blocks, seed=0xC0FFEE, offset=(k % nbatches) * n, block_size=args.block_size,
-                    slots=args.slots, iters_per_launch=args.iters, ssa_chunk=args.ssa_chunk,
-                    lanes_per_sim=args.lanes, verbose=True)
-            for k in range(min(nbatches, max(args.steps, args.warmup, 1)))]
+    n = max(threads, min(args.slice, args.sims) // threads * threads)   # simulations per step
+    nslices = max(1, args.sims // n)
+    sims = {}
+
+    def slice_sim(k):
+        """The Sim of slice k: simulations [k*n, (k+1)*n) of the workload,
+        their global indices (offset) selecting their Philox streams."""
+        k %= nslices
+        if k not in sims:
+            sims[k] = ctl.Sim(model, threads, n // threads, seed=0xC0FFEE, offset=k * n,
+                              block_size=args.block_size, slots=args.slots, iters_per_launch=args.iters,
+                              ssa_chunk=args.ssa_chunk, lanes_per_sim=args.lanes)
+        return sims[k]
+
+    for k in range(min(nslices, args.steps + args.warmup)):
+        slice_sim(k)                     # device allocation and upload, before the timed region
     for w in range(args.warmup):
-        sims[w % len(sims)].run()
-        print(f"# warmup {w} done", file=sys.stderr, flush=True)
+        slice_sim(w).run()
+        print(f"# warmup {w} done (slice {w % nslices})", file=sys.stderr, flush=True)
     fir = 0
     ker_ms = 0.0
     launches = 0
     work = 0.0
+    timed = 0
     t_start = time.perf_counter()
     for k in range(args.steps):
-        sim = sims[k % len(sims)]
+        sim = slice_sim(k)
         sim.run()
         s = sim.stats()
         fir += s["firings"]
         ker_ms += s["kernel_ms"]
         launches += s["launches"]
+        timed += n
         work += lane_ops(s, info.S, info.R, info.nL, info.nV, info.nF, SV)
-        print(f"# step {k}: firings={s['firings']} leaps={s['leaps']} ssa_steps={s['ssa_steps']} "
-              f"poisson={s['poisson']} gaps={s['gaps']} kernel_ms={s['kernel_ms']:.1f} launches={s['launches']}",
-              file=sys.stderr, flush=True)
+        print(f"# step {k} (slice {k % nslices}): firings={s['firings']} leaps={s['leaps']} "
+              f"ssa_steps={s['ssa_steps']} poisson={s['poisson']} gaps={s['gaps']} "
+              f"kernel_ms={s['kernel_ms']:.1f} launches={s['launches']}", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t_start
     last = s
-    sim_s = n * args.steps / elapsed
     kname = "tau_leap_kernel" if last["lanes_per_sim"] == 1 else f"group_kernel<{last['lanes_per_sim']}>"
     avg_launch_s = ker_ms / 1000.0 / max(launches, 1)
-    achieved = work / args.steps / max(last["launches"], 1) / avg_launch_s
-    hbm_bytes = n * (4 * info.S + 4 * info.SS * info.K + 64)   # inputs + samples + per-sim record
+    achieved = work / max(launches, 1) / avg_launch_s
+    hbm_bytes = timed * (4 * info.S + 4 * info.SS * info.K + 64)   # inputs + samples + per-sim record
+    covered = min(args.steps, nslices) * n
     out = {
         "metric": "reaction firings/sec on 1 GPU",
         "value": fir / elapsed,
@@ -163,30 +173,33 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"mass-action N={info.S} M={info.R} seed={args.seed}, {nbatches * n} simulations "
-                               f"in batches of {n} (one batch per step), {info.K} samples to "
-                               f"t_end={args.t_end:g}, Philox per simulation",
-                   "simulations": nbatches * n, "batch": n, "species": info.S, "reactions": info.R,
+        "config": {"workload": f"mass-action N={info.S} M={info.R} seed={args.seed}, {nslices * n} simulations "
+                               f"in {nslices} slices of {n}; step k runs slice k mod {nslices} from t=0 to "
+                               f"t_end={args.t_end:g} ({info.K} samples), Philox per simulation",
+                   "simulations": nslices * n, "simulations_per_step": n,
+                   "simulations_timed": timed, "distinct_simulations_timed": covered,
+                   "species": info.S, "reactions": info.R,
                    "samples": info.K, "t_end": args.t_end, "seed": args.seed},
-        "simulations_per_s": sim_s,
+        "simulations_per_s": timed / elapsed,
+        "firings_timed": fir,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
                      "unit": "Tlane-op/s", "frac": achieved / VALU_PEAK,
-                     "traffic": (MEASURED_BYTES_PER_SIM * n / max(last["launches"], 1)
+                     "traffic": (MEASURED_BYTES_PER_SIM * timed / max(launches, 1)
                                  if last["lanes_per_sim"] > 1 and args.t_end == 50.0 and args.samples == 200 else None),
                      "traffic_unit": "bytes per launch",
                      "kernel": kname, "avg_launch_ms": avg_launch_s * 1e3,
-                     "launches_per_step": last["launches"],
-                     "hbm": {"achieved_GBps": hbm_bytes / (ker_ms / 1000.0 / args.steps) / 1e9,
+                     "launches": launches,
+                     "hbm": {"achieved_GBps": hbm_bytes / (ker_ms / 1000.0) / 1e9,
                              "peak_GBps": HBM_PEAK / 1e9,
-                             "frac": hbm_bytes / (ker_ms / 1000.0 / args.steps) / HBM_PEAK}},
+                             "frac": hbm_bytes / (ker_ms / 1000.0) / HBM_PEAK}},
         "counters": {k: last[k] for k in ("firings", "leaps", "rejections", "switches", "ssa_steps",
                                           "poisson", "gaps", "absorbed", "lds_state", "slots", "block_size",
                                           "lanes_per_sim")},
     }
+    for x in sims.values():
+        x.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(net, ts, args.cpu_sims)
-    for x in sims:
-        x.close()
     print(json.dumps(out), flush=True)
 
 
@@ -214,18 +227,50 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
 
 
-def cpu_baseline(net, ts, nsims):
-    """The oracle's reference-type (double) restatement, OpenMP over simulations."""
+def host_cores():
+    """Threads for the CPU baseline: the CPUs this process may run on, at most
+    16 (a one-GPU share of the host), with nproc and the lscpu model."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model, sockets, cps, tpc = "unknown", None, None, None
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            v = v.strip()
+            if k.strip() == "Model name":
+                model = v
+            elif k.strip() == "Socket(s)":
+                sockets = int(v) if v.isdigit() else None
+            elif k.strip() == "Core(s) per socket":
+                cps = int(v) if v.isdigit() else None
+            elif k.strip() == "Thread(s) per core":
+                tpc = int(v) if v.isdigit() else None
+    except (OSError, ValueError, Exception):
+        pass
+    return min(avail, 16), {"nproc": avail, "cpu_count": os.cpu_count(), "lscpu_model": model,
+                            "sockets": sockets, "cores_per_socket": cps, "threads_per_core": tpc}
+
+
+def cpu_baseline(net, ts, nsims, locked=None, cs=None, force_ssa=False):
+    """The oracle's reference-type (double) restatement of the per-simulation
+    algorithm, OpenMP over simulations (dynamic schedule, one simulation per
+    chunk), threads bound to cores (OMP_PROC_BIND=close, OMP_PLACES=cores)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import oracle
-    threads = min(os.cpu_count() or 1, 16)
+    threads, host = host_cores()
     t0 = time.perf_counter()
     _, st = oracle.run(net["left"], net["right"], net["x0"], net["c"], ts, n=nsims, seed=0xC0FFEE,
-                       precision=oracle.REFSEM, threads=threads)
+                       locked=locked, cs=cs, force_ssa=force_ssa, precision=oracle.REFSEM, threads=threads)
     dt = time.perf_counter() - t0
     return {"value": float(st["firings"].sum()) / dt, "unit": "firings/s", "cores": threads,
             "kind": "port", "sample": f"{nsims} simulations of the same workload (full t_end), "
-                                      f"{dt:.1f}s, {st['firings'].sum()} firings",
-            "simulations_per_s": nsims / dt}
+                                      f"{dt:.1f}s, {int(st['firings'].sum())} firings",
+            "simulations_per_s": nsims / dt, "host": host}
 
 
 def _line(metric, value, unit, steps, ms, dtype, config, **extra):
@@ -234,6 +279,41 @@ def _line(metric, value, unit, steps, ms, dtype, config, **extra):
            "data": "synthetic", "config": config}
     out.update(extra)
     print(json.dumps(out), flush=True)
+
+
+def bench_c0(args):
+    """Schlogl (SURVEY's CPU-runnable configuration): the four-reaction model
+    of tests/golden/schlogl (A and B buffered), 1024 simulations to t_end=10
+    with 100 samples, one lane per simulation with its state in LDS; a step is
+    one complete run of the 1024 simulations."""
+    import cutauleaping_amd as ctl
+    left = [[1, 2, 0], [0, 3, 0], [0, 0, 1], [0, 1, 0]]
+    right = [[0, 3, 0], [1, 2, 0], [0, 1, 0], [0, 0, 1]]
+    net = {"left": left, "right": right, "x0": [100000, 250, 200000], "c": [3e-7, 1e-4, 1e-3, 3.5]}
+    ts = [round(0.1 * i, 10) for i in range(1, 101)]
+    model = ctl.Model.from_arrays(left, right, net["x0"], net["c"], ts, feed=[1, 0, 1])
+    steps = max(args.steps, 1)
+    fir, kms, launches = 0, 0.0, 0
+    with ctl.Sim(model, 64, 16, seed=0xC0FFEE, lanes_per_sim=args.lanes) as s:
+        for _ in range(args.warmup):
+            s.run()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            s.run()
+            st = s.stats()
+            fir += st["firings"]
+            kms += st["kernel_ms"]
+            launches += st["launches"]
+        dt = time.perf_counter() - t1
+    extra = {"simulations_per_s": 1024 * steps / dt, "kernel_ms_per_step": kms / steps,
+             "counters": {k: st[k] for k in ("leaps", "ssa_steps", "switches", "poisson", "gaps", "launches",
+                                             "slots", "block_size", "lds_state", "lanes_per_sim")}}
+    if not args.no_cpu:
+        extra["cpu_baseline"] = cpu_baseline(net, ts, 1024, locked=[1, 0, 1])
+    _line("reaction firings/sec on 1 GPU (Schlogl)", fir / dt, "firings/s", steps, dt * 1e3 / steps, "f32",
+          {"workload": "Schlogl (X with buffered A=1e5, B=2e5), 1024 simulations, 100 samples to t_end=10",
+           "simulations": 1024, "t_end": 10.0}, **extra)
+    model.close()
 
 
 def bench_c2(args):

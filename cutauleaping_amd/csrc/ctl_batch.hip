@@ -319,10 +319,6 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
     ctl::clear_error();
     if (!models || !replicas || !o || !out || nmodels <= 0) return ctl::fail(CTL_E_ARGS, "bad arguments");
     *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-        return ctl::fail(CTL_E_DEVICE, "no HIP device available: libctl needs an AMD GPU (gfx950)");
-    if (o->gpu < 0 || o->gpu >= ndev) return ctl::fail(CTL_E_DEVICE, "gpu ordinal out of range");
     int maxS = 0, maxR = 0;
     bool any_rates = false;
     int64_t total = 0;
@@ -339,6 +335,12 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
         total += replicas[i];
     }
     if (total > 0x7fffffffLL) return ctl::fail(CTL_E_LIMIT, "too many simulations");
+    if (o->offset < 0 || o->offset + total > 0x100000000LL)    // 32-bit Philox stream indices must not wrap
+        return ctl::fail(CTL_E_ARGS, "offset must be >= 0 and offset + total replicas <= 2^32");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return ctl::fail(CTL_E_DEVICE, "no HIP device available: libctl needs an AMD GPU (gfx950)");
+    if (o->gpu < 0 || o->gpu >= ndev) return ctl::fail(CTL_E_DEVICE, "gpu ordinal out of range");
     ctl_batch* b = new ctl_batch();
     auto bail = [&](int rc) { brelease(b); delete b; return rc; };
     b->device = o->gpu;

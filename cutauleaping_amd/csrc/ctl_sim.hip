@@ -210,6 +210,7 @@ struct ctl_sim {
     // fitness
     uint32_t* d_target = nullptr;
     int8_t* d_t2e = nullptr;
+    double* d_fit = nullptr;
 };
 
 namespace {
@@ -290,6 +291,10 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     if (m.c_rows > 1 && m.c_rows < n && n % m.c_rows)
         return ctl::fail(CTL_E_ROWS, "c_matrix rows must be 1, at least the simulation count, or divide it");
     if (o->nc < 0 || o->ssa_steps <= 0 || !(o->eps > 0.0f)) return ctl::fail(CTL_E_ARGS, "bad tunables");
+    // Philox stream indices are the 32-bit global simulation indices offset .. offset + n - 1:
+    // a range past 2^32 would wrap and alias the streams of another shard
+    if (o->offset < 0 || o->offset + n64 > 0x100000000LL)
+        return ctl::fail(CTL_E_ARGS, "offset must be >= 0 and offset + threads * blocks <= 2^32");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return ctl::fail(CTL_E_DEVICE, "no HIP device available: libctl needs an AMD GPU (gfx950)");
@@ -654,8 +659,8 @@ int ctl_fitness(const ctl_sim* sc, double* out, size_t count) {
         if ((rc = upload(s, &s->d_target, f.series.data(), f.series.size()))) return rc;
         if ((rc = upload(s, &s->d_t2e, f.thread2experiment.data(), f.thread2experiment.size()))) return rc;
     }
-    double* d_fit;
-    if ((rc = dalloc(s, &d_fit, nf))) return rc;
+    if (!s->d_fit && (rc = dalloc(s, &s->d_fit, nf))) return rc;    // once per sim, reused by later calls
+    double* d_fit = s->d_fit;
     hipLaunchKernelGGL(fitness_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s->stream, s->kp.storage,
                        s->d_target, s->d_t2e, d_fit, threads, groups, P, s->kp.K, s->kp.SS, (int)f.experiments,
                        (int)f.repetitions);

@@ -1074,7 +1074,7 @@ int orc_run(const orc_net* net, const orc_job* J, uint32_t* storage, orc_stat* s
         orc_rng* lanes = (orc_rng*)calloc((size_t)G, sizeof(orc_rng));
         int32_t* Kc = (int32_t*)malloc(sizeof(int32_t) * R);
         float* gP = (float*)malloc(sizeof(float) * 4 * (size_t)G);
-#pragma omp for schedule(dynamic, 16)
+#pragma omp for schedule(dynamic, 1)
         for (int i = 0; i < J->n; ++i) {
             simctx s;
             memset(&s, 0, sizeof(s));

@@ -121,6 +121,20 @@ def test_no_device_fails_loudly(ctl):
     assert e.value.code == ctl.CTL_E_DEVICE
 
 
+def test_offset_must_not_wrap_the_stream_index(ctl):
+    """Philox stream indices are offset .. offset + n - 1 as 32-bit words: a
+    negative offset or a range past 2^32 would alias another shard's streams,
+    so both entry points reject it before touching the device."""
+    m = ctl.Model.from_folder(os.path.join(GOLDEN, "schlogl"), 0, 64)
+    for off in (-1, (1 << 32) - 63, 1 << 40):
+        with pytest.raises(ctl.CtlError) as e:
+            ctl.Sim(m, 64, 1, offset=off)
+        assert e.value.code == ctl.CTL_E_ARGS
+        with pytest.raises(ctl.CtlError) as e:
+            ctl.Batch([m], [64], offset=off)
+        assert e.value.code == ctl.CTL_E_ARGS
+
+
 def test_cli_usage_exit_code(ctl):
     p = subprocess.run([ctl.CLI_PATH, os.path.join(GOLDEN, "schlogl"), "64"], capture_output=True, text=True)
     assert p.returncode == 255 and "please specify" in p.stdout
