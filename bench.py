@@ -6,14 +6,14 @@ mass-action network with N=100 species and M=200 reactions (order 0-2,
 [10, 1e5], mean-field int32-safety rejection), 2^20 simulations of identical
 initial state and rates with distinct Philox streams, 200 samples to t_end=50.
 
-The 2^20 simulations are processed in batches of 2^18 (--batch): a "step" is
-one complete ctl_sim_run over one batch -- every simulation of the batch from
+The 2^20 simulations are processed in slices of 2^16 (--slice): a "step" is
+one complete ctl_sim_run over one slice -- every simulation of the slice from
 t=0 to t_end, inputs resident in HBM -- and step k takes the simulations
-[k*batch, (k+1)*batch) of the workload (their global indices select their
-Philox streams).  --steps 4 covers the whole workload; the default (one
-step) keeps the GPU busy for about a minute.  A batch of 2^18 already holds
-the device full (every resident group slot runs about 20 simulations in
-turn), so the rate is that of the whole workload.
+[k*slice, (k+1)*slice) of the workload, k taken modulo the 16 slices (the
+global indices select the Philox streams).  Sixteen steps cover the whole
+workload, which is the default; warm-up steps run slices too.  A slice of
+2^16 already holds the device full (16384 resident group slots of 16 lanes
+run four simulations each in turn), so the rate is that of the workload.
 value = total firings / timed seconds (firings = accepted-leap reaction
 counts + SSA events with time <= t_end).
 
@@ -25,7 +25,8 @@ Printed JSON also carries:
   cpu_baseline the oracle's reference-type restatement (OpenMP) on a bounded
                sample of the same workload, timed on this host.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--sims N] [--batch B] [--t-end T] [--lanes G]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--sims N] [--slice B] [--t-end T] [--lanes G]
+                [--workload c0|c1|c2|c3|c4]
 """
 import argparse
 import json
@@ -71,13 +72,16 @@ def main():
     ap.add_argument("--t-end", type=float, default=50.0)
     ap.add_argument("--samples", type=int, default=200)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-sims", type=int, default=1024)
+    ap.add_argument("--cpu-sims", type=int, default=4096, help="simulations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--iters", type=int, default=0)
     ap.add_argument("--ssa-chunk", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per simulation (0: library's choice)")
+    ap.add_argument("--compact", type=int, default=0,
+                    help="one lane per simulation: iterations between regroupings by mode (<=0: off, the default)")
+    ap.add_argument("--c0-sims", type=int, default=1024, help="c0: simulations (the configuration runs 1024)")
     ap.add_argument("--workload", default="c1", choices=("c0", "c1", "c2", "c3", "c4"),
                     help="c1: the headline (default); c0: Schlogl; c2: batched uneven models; "
                          "c3: surrogate-screened sweep; c4: enzyme cascade (adaptive and force-SSA)")
@@ -98,6 +102,10 @@ def main():
               "multi-GPU sharding by input offset is documented, not launched).", file=sys.stderr)
         sys.exit(2)
     if args.workload != "c1":
+        if "--steps" not in sys.argv:       # one complete run of the other workloads unless asked
+            args.steps = 1
+        if "--warmup" not in sys.argv:
+            args.warmup = 0
         {"c0": bench_c0, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[args.workload](args)
         return
 
@@ -282,19 +290,22 @@ def _line(metric, value, unit, steps, ms, dtype, config, **extra):
 
 
 def bench_c0(args):
-    """Schlogl (SURVEY's CPU-runnable configuration): the four-reaction model
-    of tests/golden/schlogl (A and B buffered), 1024 simulations to t_end=10
-    with 100 samples, one lane per simulation with its state in LDS; a step is
-    one complete run of the 1024 simulations."""
+    """Schlogl (the reference repository's CPU-runnable configuration): the
+    four-reaction model of tests/golden/schlogl (A and B buffered), 1024
+    simulations (--c0-sims) to t_end=10 with 100 samples, one lane per
+    simulation with its state in LDS, leap-mode and SSA-mode simulations
+    regrouped into separate wavefronts (--compact); a step is one complete run
+    of the simulations."""
     import cutauleaping_amd as ctl
     left = [[1, 2, 0], [0, 3, 0], [0, 0, 1], [0, 1, 0]]
     right = [[0, 3, 0], [1, 2, 0], [0, 1, 0], [0, 0, 1]]
     net = {"left": left, "right": right, "x0": [100000, 250, 200000], "c": [3e-7, 1e-4, 1e-3, 3.5]}
     ts = [round(0.1 * i, 10) for i in range(1, 101)]
     model = ctl.Model.from_arrays(left, right, net["x0"], net["c"], ts, feed=[1, 0, 1])
+    n = max(64, args.c0_sims // 64 * 64)
     steps = max(args.steps, 1)
     fir, kms, launches = 0, 0.0, 0
-    with ctl.Sim(model, 64, 16, seed=0xC0FFEE, lanes_per_sim=args.lanes) as s:
+    with ctl.Sim(model, 64, n // 64, seed=0xC0FFEE, lanes_per_sim=args.lanes, compact_interval=args.compact) as s:
         for _ in range(args.warmup):
             s.run()
         t1 = time.perf_counter()
@@ -305,36 +316,45 @@ def bench_c0(args):
             kms += st["kernel_ms"]
             launches += st["launches"]
         dt = time.perf_counter() - t1
-    extra = {"simulations_per_s": 1024 * steps / dt, "kernel_ms_per_step": kms / steps,
+    extra = {"simulations_per_s": n * steps / dt, "kernel_ms_per_step": kms / steps,
              "counters": {k: st[k] for k in ("leaps", "ssa_steps", "switches", "poisson", "gaps", "launches",
                                              "slots", "block_size", "lds_state", "lanes_per_sim")}}
     if not args.no_cpu:
-        extra["cpu_baseline"] = cpu_baseline(net, ts, 1024, locked=[1, 0, 1])
+        extra["cpu_baseline"] = cpu_baseline(net, ts, min(n, 4096), locked=[1, 0, 1])
     _line("reaction firings/sec on 1 GPU (Schlogl)", fir / dt, "firings/s", steps, dt * 1e3 / steps, "f32",
-          {"workload": "Schlogl (X with buffered A=1e5, B=2e5), 1024 simulations, 100 samples to t_end=10",
-           "simulations": 1024, "t_end": 10.0}, **extra)
+          {"workload": f"Schlogl (X with buffered A=1e5, B=2e5), {n} simulations, 100 samples to t_end=10",
+           "simulations": n, "t_end": 10.0, "compact_interval": args.compact}, **extra)
     model.close()
 
 
 def bench_c2(args):
     """Batched uneven models: networks.batched_models(models, seed=2) -- N
     log-uniform in 10..500, M in 10..1000, 64..4096 replicas each, rates and
-    counts as C1, 10 sampled species, 100 samples to t_end -- in one
-    persistent launch (library's kernel choice: one 64-lane wavefront per
-    simulation when a model's per-lane state does not fit LDS)."""
+    counts as the headline network, 10 sampled species, 100 samples to
+    t_end -- in one batch: each model runs with the library's choice of lanes
+    per simulation, the models of one choice form a kernel class, and the
+    classes run concurrently.  A step is one complete run of the batch."""
     import cutauleaping_amd as ctl
     from cutauleaping_amd import networks
-    t_end = args.t_end if args.t_end != 50.0 else 0.5
+    t_end = args.t_end if args.t_end != 50.0 else 0.1
     t0 = time.time()
-    nets = networks.batched_models(args.models, seed=2, t_end=t_end,
+    procs = max(1, min(16, len(os.sched_getaffinity(0))))
+    nets = networks.batched_models(args.models, seed=2, t_end=t_end, processes=procs,
                                    log=lambda m: print(m, file=sys.stderr, flush=True))
-    print(f"# {len(nets)} models generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    print(f"# {len(nets)} models generated in {time.time() - t0:.1f}s on {procs} processes", file=sys.stderr,
+          flush=True)
     ts = networks.sample_times(100, t_end)
     models = [ctl.Model.from_arrays(n["left"], n["right"], n["x0"], n["c"], ts, cs=n["cs"]) for n in nets]
     reps = [n["replicas"] for n in nets]
     fir = 0
     ms = 0.0
     with ctl.Batch(models, reps, seed=0xC0FFEE, lanes_per_sim=args.lanes, verbose=True) as b:
+        lanes = np.array([b.model_lanes(i) for i in range(len(models))])
+        classes = {int(g): {"models": int((lanes == g).sum()), "simulations": int(np.asarray(reps)[lanes == g].sum())}
+                   for g in np.unique(lanes)}
+        print(f"# kernel classes (lanes per simulation: models, simulations): {classes}", file=sys.stderr, flush=True)
+        for k in range(args.warmup):
+            b.run()
         for k in range(args.steps):
             t1 = time.perf_counter()
             b.run()
@@ -347,8 +367,9 @@ def bench_c2(args):
     _line("reaction firings/sec on 1 GPU (batched uneven models)", fir / (ms / 1e3), "firings/s", args.steps,
           ms / args.steps, "f32",
           {"workload": f"{len(nets)} mass-action models, N 10..500, M 10..1000, {sims} simulations "
-                       f"(64..4096 replicas each), 100 samples to t_end={t_end:g}, one launch sequence",
-           "models": len(nets), "simulations": sims, "t_end": t_end},
+                       f"(64..4096 replicas each), 100 samples to t_end={t_end:g}, one batch of concurrent "
+                       f"per-class persistent kernels", "models": len(nets), "simulations": sims, "t_end": t_end,
+           "classes": classes},
           simulations_per_s=sims * args.steps / (ms / 1e3),
           counters={k: st[k] for k in ("leaps", "ssa_steps", "switches", "poisson", "gaps", "launches",
                                        "slots", "lanes_per_sim")},

@@ -53,7 +53,7 @@ EXPORTS = (
     "ctl_batch_sim_stats", "ctl_batch_stats", "ctl_batch_stream_base", "ctl_batch_destroy",
     "ctl_mlp_create", "ctl_mlp_num_params", "ctl_mlp_params", "ctl_mlp_forward", "ctl_mlp_forward_device",
     "ctl_mlp_gradient", "ctl_mlp_set_data", "ctl_mlp_train", "ctl_mlp_destroy",
-    "ctl_model_load_feeds", "ctl_model_set_feeds",
+    "ctl_model_load_feeds", "ctl_model_set_feeds", "ctl_batch_model_lanes",
 )
 
 
@@ -79,7 +79,7 @@ class ctl_opts(C.Structure):
                 ("eps", C.c_float), ("ssa_steps", C.c_int32), ("ssa_factor", C.c_float),
                 ("seed", C.c_uint64), ("verbose", C.c_int32), ("block_size", C.c_int32),
                 ("slots", C.c_int32), ("iters_per_launch", C.c_int32), ("ssa_chunk", C.c_int32),
-                ("lanes_per_sim", C.c_int32)]
+                ("lanes_per_sim", C.c_int32), ("compact_interval", C.c_int32)]
 
 
 class ctl_stats(C.Structure):
@@ -151,6 +151,7 @@ def lib():
         L.ctl_batch_sim_stats.argtypes = [vp, i32, vp, C.c_size_t]
         L.ctl_batch_stats.argtypes = [vp, C.POINTER(ctl_stats)]
         L.ctl_batch_stream_base.argtypes = [vp, i32, C.POINTER(i64)]
+        L.ctl_batch_model_lanes.argtypes = [vp, i32, C.POINTER(i32)]
         L.ctl_batch_destroy.argtypes = [vp]
         L.ctl_batch_destroy.restype = None
         L.ctl_sim_destroy.argtypes = [vp]
@@ -269,7 +270,7 @@ class Sim:
 
     def __init__(self, model: Model, threads: int, blocks: int, *, gpu=0, offset=0, force_ssa=False,
                  seed=0x5EEDC0DE, nc=10, eps=0.03, ssa_steps=100, ssa_factor=10.0, block_size=0,
-                 slots=0, iters_per_launch=0, ssa_chunk=0, lanes_per_sim=0, verbose=False):
+                 slots=0, iters_per_launch=0, ssa_chunk=0, lanes_per_sim=0, compact_interval=0, verbose=False):
         o = default_opts()
         o.threads, o.blocks, o.gpu, o.offset = threads, blocks, gpu, offset
         o.force_ssa = int(bool(force_ssa))
@@ -277,6 +278,7 @@ class Sim:
         o.nc, o.eps, o.ssa_steps, o.ssa_factor = nc, eps, ssa_steps, ssa_factor
         o.block_size, o.slots, o.iters_per_launch, o.ssa_chunk = block_size, slots, iters_per_launch, ssa_chunk
         o.lanes_per_sim = lanes_per_sim
+        o.compact_interval = compact_interval
         o.verbose = int(bool(verbose))
         self.model = model
         self.n = threads * blocks
@@ -322,7 +324,8 @@ class Sim:
         return out
 
     def moments(self, group: int):
-        """Per-group mean and standard deviation, [n/group, K, SS] each (on the device)."""
+        """Per-group mean and standard deviation, [n/group, K, SS] each (on the
+        device)."""
         g = self.n // group
         mean = np.zeros((g, self.K, self.SS), np.float64)
         sd = np.zeros_like(mean)
@@ -389,6 +392,12 @@ class Batch:
         b = C.c_int64()
         _check(lib().ctl_batch_stream_base(self._h, model, C.byref(b)))
         return b.value
+
+    def model_lanes(self, model: int) -> int:
+        """Lanes per simulation the model runs with (its kernel class)."""
+        g = C.c_int32()
+        _check(lib().ctl_batch_model_lanes(self._h, model, C.byref(g)))
+        return g.value
 
     def stats(self) -> dict:
         s = ctl_stats()

@@ -1,12 +1,15 @@
 // ctl_batch.hip -- many independent models of uneven size in one persistent
-// launch (no counterpart in the reference, which runs one model per process).
+// launch sequence (no counterpart in the reference, which runs one model per
+// process).
 //
-// Two kernels.  batch_kernel runs one simulation per lane (small models).
-// gbatch_kernel runs the group algorithm with one 64-lane wavefront per
-// simulation and the simulation's state in LDS (large models): a wavefront
-// takes the next simulation from one global counter over all models in
-// estimated-cost order (largest first), so its model is wave-uniform and its
-// descriptor lives in scalar registers.
+// Each model runs with its own number of lanes per simulation G (chosen per
+// model, auto_lanes), and the models of one G form a class with its own
+// persistent kernel on its own stream; the classes run concurrently:
+//   G = 1      batch_kernel (below): one simulation per lane;
+//   G = 2..32  gclass_kernel<G> (ctl_gbatch.hip): 64/G groups of a wavefront
+//              run replicas of one model, the simulations' state in LDS;
+//   G = 64     gbatch_kernel (ctl_gbatch.hip): one wavefront per simulation.
+// Inside a class the work is ordered by an estimated cost, largest first.
 //
 // batch_kernel: work is split into chunks of 64 replicas of one model, ordered by an
 // estimated per-replica cost, largest first.  A wavefront takes the next
@@ -16,8 +19,9 @@
 // its lanes are idle the wavefront takes the next chunk.  Several wavefronts
 // can work on one model, and a lane never waits for a whole chunk to finish.
 // Per-lane state lives in the global slot arrays, sized for the largest
-// model.  A replica's Philox stream index is the model's base (sum of the
-// replicas of the models before it, plus the offset) plus its replica index.
+// model of the class.  A replica's Philox stream index is the model's base
+// (sum of the replicas of the models before it, plus the offset) plus its
+// replica index, whatever its class.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -150,22 +154,37 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchParams bp) {
 // ===========================================================================
 using namespace ctl;
 
-struct ctl_batch {
-    int device = 0;
+// The models of a batch that run with the same lanes per simulation G form
+// a class: one persistent kernel on its own stream (G = 1: batch_kernel,
+// G = 2..32: gclass_kernel<G>, G = 64: gbatch_kernel).  The classes of a
+// batch run concurrently.
+struct BClass {
+    int G = 1;
+    std::vector<int> models;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int nmodels = 0, ns = 0, block = 256, grid = 0, nchunks = 0;
+    int block = 64, grid = 0, ns = 0;        // ns: resident lanes (G = 1) or groups (G > 1)
+    size_t smem = 0;
+    int64_t total = 0;                        // replicas of the class's models
+    uint32_t* d_done = nullptr;
+    BatchParams bp{};
+    GBatchParams gp{};
+    GClassParams cp{};
+    const SlotState* st = nullptr;           // G = 1: device copy of the slot arrays' pointers
+    bool finished = false;
+};
+
+struct ctl_batch {
+    int device = 0;
+    int nmodels = 0;
     int64_t total = 0;
     std::vector<KParams> desc;                 // host copies (device pointers inside)
     std::vector<int64_t> reps, base;
-    std::vector<int> K, SS;
+    std::vector<int> K, SS, lanes;
     KParams* d_desc = nullptr;
-    BatchParams bp{};
-    int lanes = 1;                             // 1: batch_kernel; 64: gbatch_kernel
+    uint32_t* d_rep_next = nullptr;            // per-model replica counters
+    std::vector<BClass> classes;
     int verbose = 0;
-    GBatchParams gp{};
-    size_t smem = 0;
-    uint32_t* d_done = nullptr;
     std::vector<void*> allocs;
     bool ran = false;
     ctl_stats stats{};
@@ -206,15 +225,30 @@ int bupload(ctl_batch* b, T** p, const T* src, size_t count) {
 void brelease(ctl_batch* b) {
     for (void* q : b->allocs) (void)hipFree(q);
     b->allocs.clear();
-    if (b->ev0) (void)hipEventDestroy(b->ev0);
-    if (b->ev1) (void)hipEventDestroy(b->ev1);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
-    b->ev0 = b->ev1 = nullptr;
-    b->stream = nullptr;
+    for (BClass& c : b->classes) {
+        if (c.ev0) (void)hipEventDestroy(c.ev0);
+        if (c.ev1) (void)hipEventDestroy(c.ev1);
+        if (c.stream) (void)hipStreamDestroy(c.stream);
+        c.ev0 = c.ev1 = nullptr;
+        c.stream = nullptr;
+    }
+}
+
+// Lanes per simulation the library picks for a model of a batch: one lane
+// when the per-lane working set fits a wavefront's 16 KB share of LDS;
+// otherwise about 16 reactions per lane (G = 4 .. 64), widened until a
+// wavefront's groups need at most 32 KB of LDS.
+int auto_lanes(const ctl::Model& m) {
+    const size_t lane_words = (size_t)2 * m.S + m.R + (m.c_rows == 1 ? 0 : m.R);
+    if (lane_words * 4 * 64 <= 16384) return 1;
+    int G = 4;
+    while (G < 64 && G * 16 < m.R) G *= 2;
+    while (G < 64 && (size_t)(64 / G) * ctl::group_lds_words(m.S, m.R, G) * 4 > 32768) G *= 2;
+    return G;
 }
 
 // Per-model device tables (the same compile products as a ctl_sim).
-int upload_model(ctl_batch* b, const ctl::Model& m, KParams& p) {
+int upload_model(ctl_batch* b, const ctl::Model& m, KParams& p, int G) {
     const int S = m.S, R = m.R;
     std::vector<uint32_t> horw(S);
     for (int i = 0; i < S; ++i) horw[i] = (uint32_t)m.hor[i] | ((uint32_t)m.hort[i] << 8);
@@ -230,9 +264,6 @@ int upload_model(ctl_batch* b, const ctl::Model& m, KParams& p) {
             list.clear();
             for (uint32_t h = m.Fptr[j]; h < m.Fptr[j + 1]; ++h)
                 for (uint32_t r : readers[m.Fent[h] & 0xffffu])
-                    if (!mark[r]) { mark[r] = 1; list.push_back(r); }
-            for (int32_t fsp : m.fs)     // feeds reset these species after every SSA step
-                for (uint32_t r : readers[fsp])
                     if (!mark[r]) { mark[r] = 1; list.push_back(r); }
             std::sort(list.begin(), list.end());
             for (uint32_t r : list) { mark[r] = 0; Dent.push_back(r); }
@@ -295,7 +326,7 @@ int upload_model(ctl_batch* b, const ctl::Model& m, KParams& p) {
         p.oDp = put(Dptr.data(), Dptr.size());
         p.oDe = put(Dent.data(), Dent.size());
         p.oCs = put(reinterpret_cast<const uint32_t*>(m.cs.data()), m.cs.size());
-        append_group_streams(T, p, 64, m, FSptr, FSent, Dent);
+        if (G > 1) append_group_streams(T, p, G, m, FSptr, FSent, Dent);
         uint32_t* Td;
         if ((rc = bupload(b, &Td, T.data(), T.size()))) return rc;
         p.T = Td;
@@ -319,9 +350,8 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
     ctl::clear_error();
     if (!models || !replicas || !o || !out || nmodels <= 0) return ctl::fail(CTL_E_ARGS, "bad arguments");
     *out = nullptr;
-    int maxS = 0, maxR = 0;
-    bool any_rates = false;
     int64_t total = 0;
+    std::vector<int> lanes(nmodels);
     for (int i = 0; i < nmodels; ++i) {
         if (!models[i] || replicas[i] <= 0) return ctl::fail(CTL_E_ARGS, "null model or non-positive replica count");
         const ctl::Model& m = models[i]->m;
@@ -329,14 +359,20 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
             (m.x0_rows > 1 && m.x0_rows < replicas[i] && replicas[i] % m.x0_rows) ||
             (m.c_rows > 1 && m.c_rows < replicas[i] && replicas[i] % m.c_rows))
             return ctl::fail(CTL_E_ROWS, "a model's input rows must be 1, at least its replicas, or divide them");
-        maxS = std::max(maxS, m.S);
-        maxR = std::max(maxR, m.R);
-        any_rates = any_rates || m.c_rows != 1;
         total += replicas[i];
+        const int G = o->lanes_per_sim > 0 ? o->lanes_per_sim : auto_lanes(m);
+        if (G < 1 || G > 64 || (G & (G - 1)))
+            return ctl::fail(CTL_E_ARGS, "lanes_per_sim must be 0 or a power of two up to 64");
+        if (G > 1 && (size_t)(64 / G) * ctl::group_lds_words(m.S, m.R, G) * 4 > 65536)
+            return ctl::fail(CTL_E_LIMIT, G == 64 ? "a model is too large for the group kernel (more than 64 KB of LDS)"
+                                                  : "a model is too large for this lanes_per_sim (more than 64 KB of LDS per wavefront)");
+        lanes[i] = G;
     }
     if (total > 0x7fffffffLL) return ctl::fail(CTL_E_LIMIT, "too many simulations");
     if (o->offset < 0 || o->offset + total > 0x100000000LL)    // 32-bit Philox stream indices must not wrap
         return ctl::fail(CTL_E_ARGS, "offset must be >= 0 and offset + total replicas <= 2^32");
+    if (o->block_size != 0 && (o->block_size % 64 != 0 || o->block_size < 0 || o->block_size > 256))
+        return ctl::fail(CTL_E_ARGS, "block_size must be 64..256, multiple of 64");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return ctl::fail(CTL_E_DEVICE, "no HIP device available: libctl needs an AMD GPU (gfx950)");
@@ -347,83 +383,118 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
     b->verbose = o->verbose;
     b->nmodels = nmodels;
     b->total = total;
+    b->lanes = lanes;
     hipError_t e = hipSetDevice(b->device);
     if (e != hipSuccess) return bail(bhip(e, "hipSetDevice"));
-    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return bail(bhip(e, "hipStreamCreate"));
-    if ((e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess)
-        return bail(bhip(e, "hipEventCreate"));
     hipDeviceProp_t prop;
     if ((e = hipGetDeviceProperties(&prop, b->device)) != hipSuccess) return bail(bhip(e, "hipGetDeviceProperties"));
-    // kernel: one lane per simulation when every model's per-lane working set
-    // fits 16 KB of LDS per wavefront, else one 64-lane wavefront per simulation
-    size_t lane_words = 0, gwords = 0;
-    for (int i = 0; i < nmodels; ++i) {
-        const ctl::Model& m = models[i]->m;
-        lane_words = std::max(lane_words, (size_t)2 * m.S + m.R + (m.c_rows == 1 ? 0 : m.R));
-        gwords = std::max(gwords, ctl::group_lds_words(m.S, m.R, 64));
-    }
-    int G = o->lanes_per_sim;
-    if (G == 0) G = lane_words * 4 * 64 <= 16384 ? 1 : 64;
-    if (G != 1 && G != 64) return bail(ctl::fail(CTL_E_ARGS, "a batch runs with lanes_per_sim 0, 1 or 64"));
-    if (G == 64 && gwords * 4 > 65536)
-        return bail(ctl::fail(CTL_E_LIMIT, "a model is too large for the group kernel (more than 64 KB of LDS)"));
-    b->lanes = G;
-    int per_cu = 0;
     int rc;
-    KParams shared{};
-    SlotState st{};
-    int32_t* X = nullptr;
-    uint32_t* A = nullptr;
-    float* Cs = nullptr;
-    int ns = 0;
-    if (G == 1) {
-        b->block = o->block_size > 0 ? o->block_size : 256;
-        if (b->block % 64 != 0 || b->block > 256) return bail(ctl::fail(CTL_E_ARGS, "block_size must be 64..256, multiple of 64"));
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)batch_kernel, b->block, 0);
-        if (e != hipSuccess || per_cu <= 0) per_cu = 1;
-        ns = o->slots > 0 ? o->slots : prop.multiProcessorCount * per_cu * b->block;
-        ns = ((ns + b->block - 1) / b->block) * b->block;
-        b->ns = ns;
-        b->grid = ns / b->block;
-        // shared slot arrays (sized for the largest model) and counters
-        if ((rc = balloc(b, &X, (size_t)2 * maxS * ns)) || (rc = balloc(b, &A, (size_t)maxR * ns)) ||
-            (any_rates && (rc = balloc(b, &Cs, (size_t)maxR * ns))))
-            return bail(rc);
-        if ((rc = balloc(b, &st.sim, ns)) || (rc = balloc(b, &st.t, ns)) || (rc = balloc(b, &st.tacc, ns)) ||
-            (rc = balloc(b, &st.cursor, ns)) || (rc = balloc(b, &st.mode, ns)) || (rc = balloc(b, &st.evc, ns)) ||
-            (rc = balloc(b, &st.ev, ns)) || (rc = balloc(b, &st.blk, ns)) || (rc = balloc(b, &st.pos, ns)) ||
-            (rc = balloc(b, &st.ssa_left, ns)) || (rc = balloc(b, &st.xb, ns)) || (rc = balloc(b, &st.firings, ns)) ||
-            (rc = balloc(b, &st.ssa_steps, ns)) || (rc = balloc(b, &st.poisson, ns)) || (rc = balloc(b, &st.gaps, ns)) ||
-            (rc = balloc(b, &st.leaps, ns)) || (rc = balloc(b, &st.rejections, ns)) || (rc = balloc(b, &st.switches, ns)))
-            return bail(rc);
-        SlotState* dst;
-        if ((rc = bupload(b, &dst, &st, 1))) return bail(rc);
-        shared.st = dst;
-    } else {
-        b->block = 64;
-        b->smem = gwords * 4;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)gbatch_kernel, 64, b->smem);
-        if (e != hipSuccess || per_cu <= 0) per_cu = 1;
-        long long groups = o->slots > 0 ? (long long)o->slots : (long long)prop.multiProcessorCount * per_cu;
-        if (groups > total) groups = total;
-        b->ns = (int)(groups * 64);
-        b->grid = (int)groups;
-        GBatchParams& g = b->gp;
-        g.ngroups = (int)groups;
-        g.maxS = maxS;
-        g.total = total;
-        if ((rc = balloc(b, &g.slot_model, groups)) || (rc = balloc(b, &g.slot_sim, groups)) ||
-            (rc = balloc(b, &g.t, groups)) || (rc = balloc(b, &g.cursor, groups)) || (rc = balloc(b, &g.evc, groups)) ||
-            (rc = balloc(b, &g.firings, groups)) || (rc = balloc(b, &g.ssa_steps, groups)) ||
-            (rc = balloc(b, &g.leaps, groups)) || (rc = balloc(b, &g.rejections, groups)) ||
-            (rc = balloc(b, &g.switches, groups)) || (rc = balloc(b, &g.poisson, (size_t)groups * 64)) ||
-            (rc = balloc(b, &g.gaps, (size_t)groups * 64)) || (rc = balloc(b, &g.X, (size_t)groups * maxS)) ||
-            (rc = balloc(b, &g.next, 1)))
-            return bail(rc);
-        g.iters = o->iters_per_launch > 0 ? o->iters_per_launch : 65536;
+    // classes, ascending G
+    for (int G = 1; G <= 64; G *= 2) {
+        BClass c;
+        c.G = G;
+        for (int i = 0; i < nmodels; ++i)
+            if (lanes[i] == G) { c.models.push_back(i); c.total += replicas[i]; }
+        if (!c.models.empty()) b->classes.push_back(c);
     }
-    ns = b->ns;
-    if ((rc = balloc(b, &b->d_done, 1))) return bail(rc);
+    if ((rc = balloc(b, &b->d_rep_next, nmodels))) return bail(rc);
+    std::vector<int> cls(nmodels, -1);
+    for (size_t k = 0; k < b->classes.size(); ++k) {
+        BClass& c = b->classes[k];
+        for (int i : c.models) cls[i] = (int)k;
+        if ((e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking)) != hipSuccess)
+            return bail(bhip(e, "hipStreamCreate"));
+        if ((e = hipEventCreate(&c.ev0)) != hipSuccess || (e = hipEventCreate(&c.ev1)) != hipSuccess)
+            return bail(bhip(e, "hipEventCreate"));
+        if ((rc = balloc(b, &c.d_done, 1))) return bail(rc);
+        int maxS = 0, maxR = 0;
+        size_t gwords = 0;
+        bool any_rates = false;
+        for (int i : c.models) {
+            const ctl::Model& m = models[i]->m;
+            maxS = std::max(maxS, m.S);
+            maxR = std::max(maxR, m.R);
+            any_rates = any_rates || m.c_rows != 1;
+            if (c.G > 1) gwords = std::max(gwords, ctl::group_lds_words(m.S, m.R, c.G));
+        }
+        int per_cu = 0;
+        if (c.G == 1) {
+            c.block = o->block_size > 0 ? o->block_size : 256;
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)batch_kernel, c.block, 0);
+            if (e != hipSuccess || per_cu <= 0) per_cu = 1;
+            int ns = o->slots > 0 ? o->slots : prop.multiProcessorCount * per_cu * c.block;
+            ns = ((ns + c.block - 1) / c.block) * c.block;
+            c.ns = ns;
+            c.grid = ns / c.block;
+            BatchParams& bp = c.bp;
+            SlotState st{};
+            if ((rc = balloc(b, &bp.X, (size_t)2 * maxS * ns)) || (rc = balloc(b, &bp.A, (size_t)maxR * ns)) ||
+                (any_rates && (rc = balloc(b, &bp.Cs, (size_t)maxR * ns))))
+                return bail(rc);
+            if ((rc = balloc(b, &st.sim, ns)) || (rc = balloc(b, &st.t, ns)) || (rc = balloc(b, &st.tacc, ns)) ||
+                (rc = balloc(b, &st.cursor, ns)) || (rc = balloc(b, &st.mode, ns)) || (rc = balloc(b, &st.evc, ns)) ||
+                (rc = balloc(b, &st.ev, ns)) || (rc = balloc(b, &st.blk, ns)) || (rc = balloc(b, &st.pos, ns)) ||
+                (rc = balloc(b, &st.ssa_left, ns)) || (rc = balloc(b, &st.xb, ns)) || (rc = balloc(b, &st.firings, ns)) ||
+                (rc = balloc(b, &st.ssa_steps, ns)) || (rc = balloc(b, &st.poisson, ns)) || (rc = balloc(b, &st.gaps, ns)) ||
+                (rc = balloc(b, &st.leaps, ns)) || (rc = balloc(b, &st.rejections, ns)) || (rc = balloc(b, &st.switches, ns)))
+                return bail(rc);
+            SlotState* dst;
+            if ((rc = bupload(b, &dst, &st, 1))) return bail(rc);
+            c.st = dst;
+            bp.ns = ns;
+            bp.iters = o->iters_per_launch > 0 ? o->iters_per_launch : 4096;
+            bp.rep_next = b->d_rep_next;
+            if ((rc = balloc(b, &bp.chunk_next, 1)) || (rc = balloc(b, &bp.slot_model, ns))) return bail(rc);
+        } else if (c.G == 64) {
+            c.block = 64;
+            c.smem = gwords * 4;
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)gbatch_kernel, 64, c.smem);
+            if (e != hipSuccess || per_cu <= 0) per_cu = 1;
+            long long groups = o->slots > 0 ? (long long)o->slots : (long long)prop.multiProcessorCount * per_cu;
+            if (groups > c.total) groups = c.total;
+            c.ns = (int)groups;
+            c.grid = (int)groups;
+            GBatchParams& g = c.gp;
+            g.ngroups = (int)groups;
+            g.maxS = maxS;
+            g.total = c.total;
+            if ((rc = balloc(b, &g.slot_model, groups)) || (rc = balloc(b, &g.slot_sim, groups)) ||
+                (rc = balloc(b, &g.t, groups)) || (rc = balloc(b, &g.cursor, groups)) || (rc = balloc(b, &g.evc, groups)) ||
+                (rc = balloc(b, &g.firings, groups)) || (rc = balloc(b, &g.ssa_steps, groups)) ||
+                (rc = balloc(b, &g.leaps, groups)) || (rc = balloc(b, &g.rejections, groups)) ||
+                (rc = balloc(b, &g.switches, groups)) || (rc = balloc(b, &g.poisson, (size_t)groups * 64)) ||
+                (rc = balloc(b, &g.gaps, (size_t)groups * 64)) || (rc = balloc(b, &g.X, (size_t)groups * maxS)) ||
+                (rc = balloc(b, &g.next, 1)))
+                return bail(rc);
+            g.iters = o->iters_per_launch > 0 ? o->iters_per_launch : 65536;
+        } else {
+            const int gpw = 64 / c.G;                       // groups per wavefront (= workgroup)
+            c.block = 64;
+            c.smem = (size_t)gpw * gwords * 4;
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ctl::gclass_kernel_fn(c.G), 64, c.smem);
+            if (e != hipSuccess || per_cu <= 0) per_cu = 1;
+            long long waves = o->slots > 0 ? ((long long)o->slots + gpw - 1) / gpw
+                                           : (long long)prop.multiProcessorCount * per_cu;
+            const long long need = (c.total + gpw - 1) / gpw;
+            if (waves > need) waves = need;
+            c.grid = (int)waves;
+            c.ns = (int)(waves * gpw);
+            GClassParams& g = c.cp;
+            g.nwaves = (int)waves;
+            g.maxS = maxS;
+            g.gwords = (int)gwords;
+            const size_t ng = (size_t)c.ns;
+            if ((rc = balloc(b, &g.wave_model, waves)) || (rc = balloc(b, &g.slot_sim, ng)) ||
+                (rc = balloc(b, &g.t, ng)) || (rc = balloc(b, &g.cursor, ng)) || (rc = balloc(b, &g.evc, ng)) ||
+                (rc = balloc(b, &g.firings, ng)) || (rc = balloc(b, &g.ssa_steps, ng)) ||
+                (rc = balloc(b, &g.leaps, ng)) || (rc = balloc(b, &g.rejections, ng)) ||
+                (rc = balloc(b, &g.switches, ng)) || (rc = balloc(b, &g.poisson, ng * c.G)) ||
+                (rc = balloc(b, &g.gaps, ng * c.G)) || (rc = balloc(b, &g.X, ng * maxS)) ||
+                (rc = balloc(b, &g.chunk_next, 1)))
+                return bail(rc);
+            g.iters = o->iters_per_launch > 0 ? o->iters_per_launch : 65536;
+        }
+    }
     b->desc.resize(nmodels);
     b->reps.assign(replicas, replicas + nmodels);
     b->base.resize(nmodels);
@@ -433,8 +504,9 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
     int64_t acc = 0;
     for (int i = 0; i < nmodels; ++i) {
         const ctl::Model& m = models[i]->m;
-        KParams p = shared;
-        if ((rc = upload_model(b, m, p))) return bail(rc);
+        const BClass& c = b->classes[cls[i]];
+        KParams p{};
+        if ((rc = upload_model(b, m, p, lanes[i]))) return bail(rc);
         p.n = (int)replicas[i];
         p.x0_div = (m.x0_rows <= 1 || m.x0_rows >= p.n) ? 1 : (int)(p.n / m.x0_rows);
         p.c_div = (m.c_rows <= 1 || m.c_rows >= p.n) ? 1 : (int)(p.n / m.c_rows);
@@ -451,13 +523,16 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
         p.force_ssa = o->force_ssa ? 1 : 0;
         p.ssa_chunk = o->ssa_chunk > 0 ? o->ssa_chunk : 10;
         p.iters = 0;
-        p.ns = ns;
-        p.X = X; p.A = A; p.Cs = Cs;
+        if (c.G == 1) {
+            p.ns = c.ns;
+            p.X = c.bp.X; p.A = c.bp.A; p.Cs = c.bp.Cs;
+            p.st = c.st;
+        }
         if ((rc = balloc(b, &p.storage, (size_t)p.n * p.K * p.SS)) || (rc = balloc(b, &p.simstat, p.n)) ||
             (rc = balloc(b, &p.xfinal, (size_t)p.n * p.S)) || (rc = balloc(b, &p.tfinal, p.n)))
             return bail(rc);
-        p.next_sim = nullptr;
-        p.done = b->d_done;
+        p.next_sim = b->d_rep_next + i;       // the group classes pull replicas through gfetch
+        p.done = c.d_done;
         b->desc[i] = p;
         b->K[i] = p.K;
         b->SS[i] = p.SS;
@@ -465,50 +540,49 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
         cost[i] = (double)(2 * m.R + 5 * m.nL + 5 * m.nV + m.nF + m.S);
     }
     if ((rc = bupload(b, &b->d_desc, b->desc.data(), b->desc.size()))) return bail(rc);
-    std::vector<int> order(nmodels);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] * b->reps[a] > cost[c] * b->reps[c]; });
-    b->stats.lanes_per_sim = G;
-    if (G != 1) {                        // one global counter over the models in cost order
-        std::vector<int64_t> cum(nmodels + 1, 0);
-        for (int k = 0; k < nmodels; ++k) cum[k + 1] = cum[k] + b->reps[order[k]];
-        GBatchParams& g = b->gp;
-        int32_t* d_order;
-        int64_t* d_cum;
-        if ((rc = bupload(b, &d_order, order.data(), order.size())) || (rc = bupload(b, &d_cum, cum.data(), cum.size())))
-            return bail(rc);
-        g.desc = b->d_desc;
-        g.order = d_order;
-        g.cum = d_cum;
-        g.nmodels = nmodels;
-        b->stats.slots = b->ns;
-        b->stats.block_size = 64;
-        b->stats.lds_state = 1;
-        *out = b;
-        return 0;
+    int maxG = 0;
+    bool uniform = true;
+    for (BClass& c : b->classes) {
+        std::vector<int> order = c.models;     // largest work first
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return cost[x] * b->reps[x] > cost[y] * b->reps[y]; });
+        if (c.G == 64) {                       // one counter over the class's models in cost order
+            std::vector<int64_t> cum(order.size() + 1, 0);
+            for (size_t k = 0; k < order.size(); ++k) cum[k + 1] = cum[k] + b->reps[order[k]];
+            GBatchParams& g = c.gp;
+            int32_t* d_order;
+            int64_t* d_cum;
+            if ((rc = bupload(b, &d_order, order.data(), order.size())) || (rc = bupload(b, &d_cum, cum.data(), cum.size())))
+                return bail(rc);
+            g.desc = b->d_desc;
+            g.order = d_order;
+            g.cum = d_cum;
+            g.nmodels = (int)order.size();
+        } else {                               // chunks of one wavefront's worth of replicas
+            const int per = c.G == 1 ? 64 : 64 / c.G;
+            std::vector<int32_t> chunks;
+            for (int i : order)
+                for (int64_t r = 0; r < b->reps[i]; r += per) chunks.push_back(i);
+            int32_t* d_chunks;
+            if ((rc = bupload(b, &d_chunks, chunks.data(), chunks.size()))) return bail(rc);
+            if (c.G == 1) {
+                c.bp.desc = b->d_desc;
+                c.bp.chunk_model = d_chunks;
+                c.bp.nchunks = (int)chunks.size();
+            } else {
+                c.cp.desc = b->d_desc;
+                c.cp.chunk_model = d_chunks;
+                c.cp.nchunks = (int)chunks.size();
+            }
+        }
+        if (maxG && c.G != maxG) uniform = false;
+        maxG = std::max(maxG, c.G);
     }
-    std::vector<int32_t> chunks;
-    for (int i : order)
-        for (int64_t r = 0; r < b->reps[i]; r += 64) chunks.push_back(i);
-    b->nchunks = (int)chunks.size();
-    int32_t* d_chunks;
-    uint32_t *d_next, *d_rep;
-    int32_t* d_slot_model;
-    if ((rc = bupload(b, &d_chunks, chunks.data(), chunks.size())) || (rc = balloc(b, &d_next, 1)) ||
-        (rc = balloc(b, &d_rep, nmodels)) || (rc = balloc(b, &d_slot_model, ns)))
-        return bail(rc);
-    BatchParams& bp = b->bp;
-    bp.desc = b->d_desc;
-    bp.chunk_model = d_chunks;
-    bp.nchunks = b->nchunks;
-    bp.chunk_next = d_next;
-    bp.rep_next = d_rep;
-    bp.slot_model = d_slot_model;
-    bp.ns = ns;
-    bp.iters = o->iters_per_launch > 0 ? o->iters_per_launch : 4096;
-    bp.X = X; bp.A = A; bp.Cs = Cs;
-    b->stats.slots = ns;
-    b->stats.block_size = b->block;
+    b->stats.lanes_per_sim = uniform ? maxG : 0;
+    b->stats.slots = 0;
+    for (BClass& c : b->classes) b->stats.slots += c.ns;
+    b->stats.block_size = b->classes.size() == 1 ? b->classes[0].block : 0;
+    b->stats.lds_state = b->classes.size() == 1 && b->classes[0].G == 1 ? 0 : 1;
     *out = b;
     return 0;
 }
@@ -518,43 +592,74 @@ int ctl_batch_run(ctl_batch* b) {
     if (!b) return ctl::fail(CTL_E_ARGS, "null batch");
     BHIP(hipSetDevice(b->device), "hipSetDevice");
     const auto w0 = std::chrono::steady_clock::now();
-    for (int i = 0; i < b->nmodels; ++i) {
-        const KParams& p = b->desc[i];
-        BHIP(hipMemsetAsync(p.storage, 0, sizeof(uint32_t) * (size_t)p.n * p.K * p.SS, b->stream), "memset");
+    for (BClass& c : b->classes) {
+        for (int i : c.models) {
+            const KParams& p = b->desc[i];
+            BHIP(hipMemsetAsync(p.storage, 0, sizeof(uint32_t) * (size_t)p.n * p.K * p.SS, c.stream), "memset");
+        }
+        if (c.G == 1) {
+            BHIP(hipMemsetAsync(c.bp.slot_model, 0xff, sizeof(int32_t) * c.ns, c.stream), "memset");
+            BHIP(hipMemsetAsync(c.bp.chunk_next, 0, sizeof(uint32_t), c.stream), "memset");
+        } else if (c.G == 64) {
+            BHIP(hipMemsetAsync(c.gp.slot_sim, 0xff, sizeof(int32_t) * c.gp.ngroups, c.stream), "memset");
+            BHIP(hipMemsetAsync(c.gp.slot_model, 0, sizeof(int32_t) * c.gp.ngroups, c.stream), "memset");
+            BHIP(hipMemsetAsync(c.gp.next, 0, sizeof(uint32_t), c.stream), "memset");
+        } else {
+            BHIP(hipMemsetAsync(c.cp.wave_model, 0xff, sizeof(int32_t) * c.cp.nwaves, c.stream), "memset");
+            BHIP(hipMemsetAsync(c.cp.slot_sim, 0xff, sizeof(int32_t) * c.ns, c.stream), "memset");
+            BHIP(hipMemsetAsync(c.cp.chunk_next, 0, sizeof(uint32_t), c.stream), "memset");
+        }
+        BHIP(hipMemsetAsync(c.d_done, 0, sizeof(uint32_t), c.stream), "memset");
+        c.finished = false;
     }
-    if (b->lanes == 1) {
-        BHIP(hipMemsetAsync(b->bp.slot_model, 0xff, sizeof(int32_t) * b->ns, b->stream), "memset");
-        BHIP(hipMemsetAsync(b->bp.rep_next, 0, sizeof(uint32_t) * b->nmodels, b->stream), "memset");
-        BHIP(hipMemsetAsync(b->bp.chunk_next, 0, sizeof(uint32_t), b->stream), "memset");
-    } else {
-        BHIP(hipMemsetAsync(b->gp.slot_sim, 0xff, sizeof(int32_t) * b->gp.ngroups, b->stream), "memset");
-        BHIP(hipMemsetAsync(b->gp.slot_model, 0, sizeof(int32_t) * b->gp.ngroups, b->stream), "memset");
-        BHIP(hipMemsetAsync(b->gp.next, 0, sizeof(uint32_t), b->stream), "memset");
-    }
-    BHIP(hipMemsetAsync(b->d_done, 0, sizeof(uint32_t), b->stream), "memset");
+    // replica counters of every model (before any class starts)
+    BHIP(hipMemset(b->d_rep_next, 0, sizeof(uint32_t) * b->nmodels), "memset");
     ctl_stats& S = b->stats;
     S.launches = 0;
-    float total_ms = 0.0f;
-    uint32_t done = 0;
-    while (done < (uint32_t)b->total) {
-        BHIP(hipEventRecord(b->ev0, b->stream), "hipEventRecord");
-        if (b->lanes == 1)
-            hipLaunchKernelGGL(batch_kernel, dim3(b->grid), dim3(b->block), 0, b->stream, b->bp);
-        else
-            hipLaunchKernelGGL(gbatch_kernel, dim3(b->grid), dim3(64), b->smem, b->stream, b->gp);
-        BHIP(hipGetLastError(), "batch kernel launch");
-        BHIP(hipEventRecord(b->ev1, b->stream), "hipEventRecord");
-        BHIP(hipMemcpyAsync(&done, b->d_done, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream), "hipMemcpy D2H");
-        BHIP(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
-        float ms = 0.0f;
-        BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1), "hipEventElapsedTime");
-        total_ms += ms;
-        if (++S.launches >= (1 << 20)) return ctl::fail(CTL_E_STATE, "batch did not finish");
-        if (b->verbose)
-            std::fprintf(stderr, " * launch %d: %u/%lld simulations done (%.3f ms)\n", S.launches, done,
-                         (long long)b->total, ms);
+    // the classes run concurrently: each round launches every unfinished
+    // class on its own stream, then waits for all of them
+    const auto k0 = std::chrono::steady_clock::now();
+    std::vector<uint32_t> done(b->classes.size(), 0);
+    for (;;) {
+        bool any = false;
+        for (size_t k = 0; k < b->classes.size(); ++k) {
+            BClass& c = b->classes[k];
+            if (c.finished) continue;
+            any = true;
+            BHIP(hipEventRecord(c.ev0, c.stream), "hipEventRecord");
+            if (c.G == 1)
+                hipLaunchKernelGGL(batch_kernel, dim3(c.grid), dim3(c.block), 0, c.stream, c.bp);
+            else if (c.G == 64)
+                hipLaunchKernelGGL(gbatch_kernel, dim3(c.grid), dim3(64), c.smem, c.stream, c.gp);
+            else
+                switch (c.G) {
+                    case 2: hipLaunchKernelGGL(gclass_kernel<2>, dim3(c.grid), dim3(64), c.smem, c.stream, c.cp); break;
+                    case 4: hipLaunchKernelGGL(gclass_kernel<4>, dim3(c.grid), dim3(64), c.smem, c.stream, c.cp); break;
+                    case 8: hipLaunchKernelGGL(gclass_kernel<8>, dim3(c.grid), dim3(64), c.smem, c.stream, c.cp); break;
+                    case 16: hipLaunchKernelGGL(gclass_kernel<16>, dim3(c.grid), dim3(64), c.smem, c.stream, c.cp); break;
+                    default: hipLaunchKernelGGL(gclass_kernel<32>, dim3(c.grid), dim3(64), c.smem, c.stream, c.cp); break;
+                }
+            BHIP(hipGetLastError(), "batch kernel launch");
+            BHIP(hipEventRecord(c.ev1, c.stream), "hipEventRecord");
+            BHIP(hipMemcpyAsync(&done[k], c.d_done, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream), "hipMemcpy D2H");
+            ++S.launches;
+        }
+        if (!any) break;
+        for (size_t k = 0; k < b->classes.size(); ++k) {
+            BClass& c = b->classes[k];
+            if (c.finished) continue;
+            BHIP(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+            if (done[k] >= (uint32_t)c.total) c.finished = true;
+            if (b->verbose) {
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, c.ev0, c.ev1);
+                std::fprintf(stderr, " * class G=%d launch: %u/%lld simulations done (%.3f ms)\n", c.G, done[k],
+                             (long long)c.total, ms);
+            }
+        }
+        if (S.launches >= (1 << 20)) return ctl::fail(CTL_E_STATE, "batch did not finish");
     }
-    S.kernel_ms = total_ms;
+    S.kernel_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - k0).count();
     S.firings = S.leaps = S.rejections = S.switches = S.ssa_steps = S.poisson = S.gaps = S.absorbed = 0;
     for (int i = 0; i < b->nmodels; ++i) {
         std::vector<ctl_sim_stat> st(b->desc[i].n);
@@ -595,6 +700,12 @@ int ctl_batch_sim_stats(const ctl_batch* b, int32_t model, ctl_sim_stat* out, si
 int ctl_batch_stats(const ctl_batch* b, ctl_stats* out) {
     if (!b || !out) return ctl::fail(CTL_E_ARGS, "null argument");
     *out = b->stats;
+    return 0;
+}
+
+int ctl_batch_model_lanes(const ctl_batch* b, int32_t model, int32_t* lanes) {
+    if (!b || !lanes || model < 0 || model >= b->nmodels) return ctl::fail(CTL_E_ARGS, "bad arguments");
+    *lanes = b->lanes[model];
     return 0;
 }
 

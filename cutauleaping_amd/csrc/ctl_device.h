@@ -131,29 +131,35 @@ __device__ __forceinline__ float dexpf(float x) {
     return p * __uint_as_float((uint32_t)(k + 127) << 23);
 }
 
-// log(1 - v), v in [0, 1): the series above with f = -v exactly while
-// 1 - v > sqrt(1/2), so small arguments keep their full relative precision.
-__device__ __forceinline__ float dlog1m(float v) {
-    if (v < 0x1.2bec34p-2f) {
-        const float f = -v;
-        const float s = f / (2.0f + f);
-        const float z = s * s;
-        float R = fmaf(z, 0x1.c71c72p-3f, 0x1.24924ap-2f);
-        R = fmaf(z, R, 0x1.99999ap-2f);
-        R = fmaf(z, R, 0x1.555556p-1f);
-        R = R * z;
-        const float hfsq = 0.5f * f * f;
-        return f - (hfsq - s * (hfsq + R));
-    }
-    return dlogf(1.0f - v);
-}
-
 // Exponential(1) variate -log(1 - V): V = r 2^-32 below 2^-8 (exact) and
 // (r >> 8) 2^-24 above, so short gaps keep 32-bit resolution and V < 1.
+// log(1 - V) is dlogf's series with f = -V exactly while 1 - V > sqrt(1/2)
+// (small arguments keep their full relative precision), else dlogf(1 - V).
+// The two cases share one series and one division without a branch (the
+// lanes of a wavefront draw different V): f and the exponent are selected,
+// and each case ends with exactly the additions of its own formula.
 __device__ __forceinline__ float exp_rv(Rng& g) {
     const uint32_t r = rng_u32(g);
     const float v = r < 0x01000000u ? (float)r * 0x1p-32f : (float)(r >> 8) * 0x1p-24f;
-    return -dlog1m(v);
+    const bool small = v < 0x1.2bec34p-2f;
+    // dlogf's reduction of x = 1 - v (exact on the large branch)
+    const uint32_t ix = __float_as_uint(1.0f - v);
+    int e = (int)(ix >> 23) - 127;
+    uint32_t mb = (ix & 0x007fffffu) | 0x3f800000u;
+    if (mb > 0x3fb504f3u) { mb -= 0x00800000u; e += 1; }
+    const float f = small ? -v : __uint_as_float(mb) - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    float R = fmaf(z, 0x1.c71c72p-3f, 0x1.24924ap-2f);
+    R = fmaf(z, R, 0x1.99999ap-2f);
+    R = fmaf(z, R, 0x1.555556p-1f);
+    R = R * z;
+    const float hfsq = 0.5f * f * f;
+    const float t = s * (hfsq + R);
+    const float ef = (float)e;
+    const float lsmall = f - (hfsq - t);
+    const float llarge = ef * 0x1.62e4p-1f + (f - (hfsq - (t + ef * 0x1.7f7d1cp-20f)));
+    return -(small ? lsmall : llarge);
 }
 
 // ---------------------------------------------------------------------------

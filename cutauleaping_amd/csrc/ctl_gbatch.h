@@ -46,4 +46,40 @@ static __device__ __forceinline__ KParams desc_of_b(const KParams* desc, int m) 
 
 __global__ void gbatch_kernel(GBatchParams bp);
 
+// Group batch with G = 2..32 lanes per simulation: a 64-lane wavefront holds
+// 64 / G groups that all run replicas of one model at a time (the model is
+// wave-uniform, so its tables stay scalar loads).  The wavefront takes the
+// next chunk of the class's chunk list (chunks name models, largest work
+// first); its groups then pull replicas of that model from the model's own
+// counter (KParams::next_sim) until it is exhausted, and when no group of
+// the wavefront holds a replica the wavefront takes the next chunk.
+struct GClassParams {
+    const KParams* desc;        // per-model descriptors (device)
+    const int32_t* chunk_model; // chunk -> model
+    int nchunks;
+    uint32_t* chunk_next;
+    int nwaves;                 // resident wavefronts (one per workgroup)
+    int iters;
+    int maxS;
+    int gwords;                 // LDS words per group (the class's largest model)
+    // parked state
+    int32_t* wave_model;        // [wave] model in progress, -1 none
+    int32_t* slot_sim;          // [group slot] replica, -1 none, -2 the model is exhausted for it
+    double* t;
+    uint32_t* cursor;
+    uint32_t* evc;
+    uint64_t* firings;
+    uint64_t* ssa_steps;
+    uint32_t* leaps;
+    uint32_t* rejections;
+    uint32_t* switches;
+    uint64_t* poisson;          // [slot][G]
+    uint64_t* gaps;             // [slot][G]
+    int32_t* X;                 // [slot][maxS]
+};
+
+template <int G>
+__global__ void gclass_kernel(GClassParams bp);
+const void* gclass_kernel_fn(int G);
+
 }  // namespace ctl

@@ -11,6 +11,12 @@
 //     Kc  int32 [Rp]     counts drawn for the current leap attempt
 //     Q   uint16[R]      reactions queued for a Poisson draw (lambda >= 1), in
 //                        the trial-state half of X while it is free
+//     F   uint32[2S+1]   reactant factors of the current state (reactions of
+//                        at most two reactant entries of multiplicity <= 2):
+//                        F[s] = x_s, F[S+s] = x_s (x_s - 1) / 2 as FP32 bits,
+//                        each with its criticality in bit 31, F[2S] = 1.0; in
+//                        the Kc region, which is free outside the leap's
+//                        count and update phases
 // so every table-driven access is an LDS access instead of a dependent HBM
 // load.  Group-wide values (a0, tau, the chosen reaction) are computed with
 // DPP row operations and lane reads and are identical in every lane of the
@@ -212,6 +218,8 @@ struct GView {
     uint32_t* A;             // [Rp] (entries R .. Rp-1 stay 0)
     int32_t* Kc;             // [Rp]
     uint16_t* Qx;            // [R] own queue region, or null: the queue lives in the trial state
+    uint32_t* F;             // [2S+1] reactant factors (the Kc region when it is large enough)
+    uint32_t* qn;            // length of the Poisson queue (LDS atomic counter)
 };
 
 // Padded reaction count of the LDS arrays: R + 1 rounded up to a multiple of
@@ -220,12 +228,18 @@ struct GView {
 // of the tau' stream point at.
 __host__ __device__ __forceinline__ int group_rpad(int R, int G) { return (R + G) / G * G; }
 
-// LDS words of one group's simulation: X[2][S], A[Rp], Kc[Rp], and the queue
-// of lambda >= 1 reactions as 16-bit indices, which shares the trial-state
-// half of X (free until the update writes it) when R/2 <= S, else follows Kc.
+// LDS words of one group's simulation: X[2][S], A[Rp], Kc[Rp], then one
+// region shared by the queue of lambda >= 1 reactions (16-bit indices; it
+// lives in the trial-state half of X, free until the update writes it, when
+// R/2 <= S) and the factor table F[2S+1] (in Kc when 2S+1 <= Rp): the queue
+// is used only by the count phase, the table only before it and in SSA;
+// last, the queue's length.
 __host__ __device__ __forceinline__ size_t group_lds_words(int S, int R, int G) {
+    const size_t Rp = (size_t)group_rpad(R, G);
     const size_t q = ((size_t)R + 1) / 2;
-    return (size_t)2 * S + (size_t)2 * group_rpad(R, G) + (q > (size_t)S ? q : 0);
+    const size_t qx = q > (size_t)S ? q : 0;
+    const size_t fx = (size_t)2 * S + 1 > Rp ? (size_t)2 * S + 1 : 0;
+    return (size_t)2 * S + 2 * Rp + (qx > fx ? qx : fx) + 1;
 }
 
 __device__ __forceinline__ GView group_view(uint32_t* base, int S, int R, int G) {
@@ -235,7 +249,30 @@ __device__ __forceinline__ GView group_view(uint32_t* base, int S, int R, int G)
     V.A = (uint32_t*)(V.Xb + 2 * S);
     V.Kc = (int32_t*)(V.A + Rp);
     V.Qx = ((size_t)R + 1) / 2 > (size_t)S ? (uint16_t*)(V.Kc + Rp) : nullptr;
+    V.F = 2 * S + 1 <= Rp ? (uint32_t*)V.Kc : (uint32_t*)(V.Kc + Rp);
+    V.qn = base + group_lds_words(S, R, G) - 1;
     return V;
+}
+
+// The reactant factors of species s from its count x (the arithmetic and the
+// criticality test of greactant for multiplicities 1 and 2): x and
+// x (x - 1) / 2 as FP32, bit 31 set when v <= x < nc v.
+__device__ __forceinline__ void gfactor(uint32_t* F, int S, int s, int32_t x, uint32_t nc) {
+    const float xf = (float)x;
+    const float f2 = (x < 2) ? 0.0f : (0.5f * xf) * (float)(x - 1);
+    const bool c1 = (uint32_t)x >= 1u && (uint32_t)x < nc;
+    const bool c2 = (uint32_t)x >= 2u && (uint32_t)x < nc * 2u;
+    F[s] = __float_as_uint(xf) | (c1 ? 0x80000000u : 0u);
+    F[S + s] = __float_as_uint(f2) | (c2 ? 0x80000000u : 0u);
+}
+
+// The whole factor table of state X; F[2S] = 1.0 is the factor of a padding
+// entry (multiplicity 0), never critical.
+template <int G>
+__device__ __forceinline__ void gfactors(const KParams& p, const int32_t* X, uint32_t* F, int l) {
+    const uint32_t nc = (uint32_t)p.nc;
+    for (int s = l; s < p.S; s += G) gfactor(F, p.S, s, X[s], nc);
+    if (l == 0) F[2 * p.S] = 0x3f800000u;
 }
 
 // Zeroes a group's A padding (R .. Rp-1); done when the group's model is set.
@@ -260,21 +297,6 @@ __device__ __forceinline__ void greactant(float& a, bool& c, int32_t x, int v, u
     c = c || ((uint32_t)x >= (uint32_t)v && (uint64_t)(uint32_t)x < (uint64_t)nc * (uint32_t)v);
 }
 
-// The same for a reaction with two reactant entries of multiplicity <= 2
-// (padding: multiplicity 0), branch-free: padding multiplies by 1 and is
-// never critical, and nc * v <= 2^32 - 2 fits the 32-bit compare.
-__device__ __forceinline__ void gfast2(float& a, bool& c, const int32_t* xs, const uint32_t* vs, uint32_t nc) {
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int32_t x = xs[k];
-        const uint32_t v = vs[k];
-        const float xf = (float)x;
-        const float f2 = (x < 2) ? 0.0f : (0.5f * xf) * (float)(x - 1);
-        a = a * (v == 1u ? xf : (v == 2u ? f2 : 1.0f));
-        c = c || ((uint32_t)x >= v && (uint32_t)x < nc * v);
-    }
-}
-
 // Reaction j from its own reactant list (the SSA dependents).
 __device__ __forceinline__ float gprop(const KParams& p, const int32_t* X, const float* crate, int j, bool& crit) {
     float a = crate[j];
@@ -292,25 +314,27 @@ __device__ __forceinline__ float gprop(const KParams& p, const int32_t* X, const
 // stream: the table addresses do not depend on loaded data, so the loads of a
 // round issue together.  Writes A[j] (propensity bits | critical << 31) and
 // returns the lane's partial sums of all and of the critical propensities.
+// With pfast the two entries of a reaction are indices into the factor table
+// F of the state (built by gfactors), so a reaction costs two multiplies:
+// the same factors in the same order as greactant, hence the same bits.
 template <int G>
-__device__ __forceinline__ void gprops(const KParams& p, const int32_t* X, const float* crate, uint32_t* A, int l,
-                                       float& P, float& PC) {
+__device__ __forceinline__ void gprops(const KParams& p, const int32_t* X, const uint32_t* F, const float* crate,
+                                       uint32_t* A, int l, float& P, float& PC) {
     const int nr = (p.R + G - 1) / G;
     P = 0.0f;
     PC = 0.0f;
     if (kuse(p.pfast)) {
         // two entries per reaction; four rounds per batch of loads
         const uint32_t* E = p.T + p.oPRe + l;
-        const uint32_t nc = (uint32_t)p.nc;
         for (int r0 = 0; r0 < nr; r0 += 4) {
             float cr[4];
             uint32_t e0[4], e1[4];
-            int32_t x0[4], x1[4];
+            uint32_t u0[4], u1[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int r = r0 + q, j = l + r * G;
                 cr[q] = 0.0f;
-                e0[q] = e1[q] = 0u;
+                e0[q] = e1[q] = (uint32_t)(2 * p.S);
                 if (r < nr) {
                     e0[q] = E[(2 * r) * G];
                     e1[q] = E[(2 * r + 1) * G];
@@ -319,19 +343,18 @@ __device__ __forceinline__ void gprops(const KParams& p, const int32_t* X, const
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                x0[q] = X[e0[q] & 0xffffu];
-                x1[q] = X[e1[q] & 0xffffu];
+                u0[q] = F[e0[q]];
+                u1[q] = F[e1[q]];
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int r = r0 + q, j = l + r * G;
                 if (r < nr) {                          // past R: 0 (rate 0) into A's zero padding
                     float a = cr[q];
-                    bool c = false;
-                    const int32_t xs[2] = {x0[q], x1[q]};
-                    const uint32_t vs[2] = {e0[q] >> 16, e1[q] >> 16};
-                    gfast2(a, c, xs, vs, nc);
-                    A[j] = __float_as_uint(a) | (c ? 0x80000000u : 0u);
+                    a = a * __uint_as_float(u0[q] & 0x7fffffffu);
+                    a = a * __uint_as_float(u1[q] & 0x7fffffffu);
+                    const uint32_t c = (u0[q] | u1[q]) & 0x80000000u;
+                    A[j] = __float_as_uint(a) | c;
                     P = P + a;
                     if (c) PC = PC + a;
                 }
@@ -457,7 +480,11 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
     int32_t* X = V.Xb + (L.xb ? p.S : 0);
     int32_t* X1 = V.Xb + (L.xb ? 0 : p.S);
     float P, PC;
-    gprops<G>(p, X, L.crate, V.A, l, P, PC);
+    if (kuse(p.pfast)) {
+        gfactors<G>(p, X, V.F, l);
+        gsync();
+    }
+    gprops<G>(p, X, V.F, L.crate, V.A, l, P, PC);
     gsync();
     P = gscan<G>(P, l);
     PC = gscan<G>(PC, l);
@@ -474,19 +501,43 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         const uint32_t len = TC[p.oVRm + r];
         const uint32_t spw = SPI[r * G];             // species | HOR << 16 | HORtype << 24
         float mu = 0.0f, sg = 0.0f;
-        for (uint32_t k = 0; k < len; k += 4) {      // len is a multiple of 4
-            uint32_t e[4], u[4];
+        if (kuse(p.vhalf)) {
+            // entries (coefficient as FP16 << 16 | byte offset of A[reaction]):
+            // both halves are read as operand selects, and v * v in FP32 is exact
+            const char* Ab = reinterpret_cast<const char*>(V.A);
+            for (uint32_t k = 0; k < len; k += 4) {  // len is a multiple of 4
+                uint32_t e[4];
+                int32_t u[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) e[i] = E[(k + i) * G];
+                for (int i = 0; i < 4; ++i) e[i] = E[(k + i) * G];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) u[i] = V.A[e[i] & 0xffffu];
+                for (int i = 0; i < 4; ++i) u[i] = *reinterpret_cast<const int32_t*>(Ab + (e[i] & 0xffffu));
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int v = (int)(int16_t)(e[i] >> 16);
-                // critical reactions and padding (A[R] = 0) add an exact zero
-                const float a = (u[i] & 0x80000000u) ? 0.0f : __uint_as_float(u[i]);
-                mu = mu + (float)v * a;
-                sg = sg + (float)(v * v) * a;
+                for (int i = 0; i < 4; ++i) {
+                    const float v = (float)__builtin_bit_cast(_Float16, (uint16_t)(e[i] >> 16));
+                    // a critical propensity has bit 31 set: as an integer it is
+                    // negative and the maximum with 0 gives +0; padding (A[R] = 0)
+                    // adds an exact zero too
+                    const float a = __int_as_float(u[i] > 0 ? u[i] : 0);
+                    mu = mu + v * a;
+                    sg = sg + (v * v) * a;
+                }
+            }
+        } else {
+            for (uint32_t k = 0; k < len; k += 4) {  // len is a multiple of 4
+                uint32_t e[4], u[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) e[i] = E[(k + i) * G];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) u[i] = V.A[e[i] & 0xffffu];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int v = (int)(int16_t)(e[i] >> 16);
+                    // critical reactions and padding (A[R] = 0) add an exact zero
+                    const float a = (u[i] & 0x80000000u) ? 0.0f : __uint_as_float(u[i]);
+                    mu = mu + (float)v * a;
+                    sg = sg + (float)(v * v) * a;
+                }
             }
         }
         E += len * G;
@@ -530,8 +581,10 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         // reactions with lambda >= 1, drawn afterwards by the group's lanes in
         // parallel, each from the reaction's own stream (word 3 = 2^31 | j)
         rng_event(L.lg, e);
-        uint64_t f = 0;
+        uint32_t fk = 0;                         // counts of this loop: process events and the pick
         uint16_t* Q = V.Qx ? V.Qx : (uint16_t*)X1;
+        if (l == 0) *V.qn = 0u;
+        gsync();
         double cum = 0.0, pos = (double)exp_rv(L.lg);
         L.gaps++;
         // The lane's second event position is drawn now, with the first, by
@@ -541,7 +594,6 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         // these gaps), counted as a gap when it is used.
         double pos2 = pos + (double)exp_rv(L.lg);
         bool have2 = true;
-        uint32_t nq = 0;
         const int nit = (p.R + G - 1) / G;
         for (int i = 0; i < nit; ++i) {
             const int j = l + i * G;
@@ -567,13 +619,14 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
                     L.gaps++;
                 }
                 V.Kc[j] = k;                         // a queued reaction's count is drawn below
-                f += (uint64_t)k;
+                fk += (uint32_t)k;
             }
-            const uint64_t m = gballot<G>(need);
-            if (need) Q[nq + (uint32_t)__popcll(m & ((1ull << l) - 1ull))] = (uint16_t)j;
-            nq += (uint32_t)__popcll(m);
+            // the queue's order is free: each count comes from its reaction's own stream
+            if (need) Q[atomicAdd(V.qn, 1u)] = (uint16_t)j;
         }
         gsync();
+        const uint32_t nq = *V.qn;
+        uint64_t f = fk;
         for (uint32_t qi = (uint32_t)l; qi < nq; qi += G) {
             const int j = Q[qi];
             const float lam = tau * __uint_as_float(V.A[j]);
@@ -661,9 +714,13 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
     int32_t* X = V.Xb + (L.xb ? p.S : 0);
     rng_event(L.gg, L.evc++);
     double tacc = 0.0;
+    if (kuse(p.pfast)) {
+        gfactors<G>(p, X, V.F, l);
+        gsync();
+    }
     {
         float P, PC;
-        gprops<G>(p, X, L.crate, V.A, l, P, PC);
+        gprops<G>(p, X, V.F, L.crate, V.A, l, P, PC);
     }
     gsync();
     for (int step = 0; step < p.ssa_steps; ++step) {
@@ -697,33 +754,30 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
         }
         gsync();
         const uint32_t f1 = p.T[p.oFp + j + 1];
+        const bool fac = kuse(p.pfast) != 0;
         for (uint32_t h = p.T[p.oFp + j] + (uint32_t)l; h < f1; h += G) {
             const uint32_t ent = p.T[p.oFe + h];
             const int s = (int)(ent & 0xffffu);
             const int v = (int)(int16_t)(ent >> 16);
-            X[s] = (int32_t)((uint32_t)X[s] + (uint32_t)v);
+            const int32_t x = (int32_t)((uint32_t)X[s] + (uint32_t)v);
+            X[s] = x;
+            if (fac) gfactor(V.F, p.S, s, x, (uint32_t)p.nc);   // a species occurs once per reaction
         }
         gsync();
-        if (kuse(p.nfs) > 0) {           // feeds (T.cu:2253-2258); their readers are in the dependency lists
-            for (int u = l; u < p.nfs; u += G) X[p.fs[u]] = (int32_t)L.fvrow[u];
-            gsync();
-        }
         L.ssa_steps++;
         if (tn <= p.tmax) L.firings++;
         const uint32_t d0 = p.T[p.oDp + j], d1 = p.T[p.oDp + j + 1];
-        if (kuse(p.pfast)) {
-            // dependent records {r, reactant entry 0, entry 1, 0}: one 16-byte load each
+        if (fac) {
+            // dependent records {r, factor index 0, factor index 1, 0}: one 16-byte load each
             const uint4* DR = reinterpret_cast<const uint4*>(p.T + p.oDR);
-            const uint32_t nc = (uint32_t)p.nc;
             for (uint32_t d = d0 + (uint32_t)l; d < d1; d += G) {
                 const uint4 rec = DR[d];
                 const int r = (int)rec.x;
+                const uint32_t u0 = V.F[rec.y], u1 = V.F[rec.z];
                 float a = L.crate[r];
-                const int32_t xs[2] = {X[rec.y & 0xffffu], X[rec.z & 0xffffu]};
-                const uint32_t vs[2] = {rec.y >> 16, rec.z >> 16};
-                bool c = false;
-                gfast2(a, c, xs, vs, nc);
-                V.A[r] = __float_as_uint(a) | (c ? 0x80000000u : 0u);
+                a = a * __uint_as_float(u0 & 0x7fffffffu);
+                a = a * __uint_as_float(u1 & 0x7fffffffu);
+                V.A[r] = __float_as_uint(a) | ((u0 | u1) & 0x80000000u);
             }
         } else {
             for (uint32_t d = d0 + (uint32_t)l; d < d1; d += G) {
@@ -939,7 +993,18 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
     }
     std::vector<int> rorder((size_t)((m.R + G - 1) / G) * G, -1);
     for (int j = 0; j < m.R; ++j) rorder[j] = j;          // reaction j: lane j mod G (the selection's partials)
-    rounds(rorder, m.Lptr, m.Lent, p.oPRm, p.oPRe, 1, p.pfast ? 2 : 0, 0u);
+    // pfast: a reactant entry becomes its index in the factor table F
+    // (multiplicity 1: s, 2: S + s; padding: 2S, the constant 1.0)
+    const uint32_t one = (uint32_t)(2 * m.S);
+    auto findex = [&](uint32_t e) { const uint32_t v = e >> 16, sp = e & 0xffffu;
+                                    return v == 1u ? sp : (v == 2u ? (uint32_t)m.S + sp : one); };
+    if (p.pfast) {
+        std::vector<uint32_t> fent(m.Lent.size());
+        for (size_t h = 0; h < m.Lent.size(); ++h) fent[h] = findex(m.Lent[h]);
+        rounds(rorder, m.Lptr, fent, p.oPRm, p.oPRe, 1, 2, one);
+    } else {
+        rounds(rorder, m.Lptr, m.Lent, p.oPRm, p.oPRe, 1, 0, 0u);
+    }
     p.oDR = 0;
     if (p.pfast) {                        // dependent records, 16-byte aligned
         while (T.size() % 4) T.push_back(0u);
@@ -947,8 +1012,8 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
         for (uint32_t r : Dent) {
             const uint32_t n = m.Lptr[r + 1] - m.Lptr[r];
             T.push_back(r);
-            T.push_back(n > 0 ? m.Lent[m.Lptr[r]] : 0u);
-            T.push_back(n > 1 ? m.Lent[m.Lptr[r] + 1] : 0u);
+            T.push_back(n > 0 ? findex(m.Lent[m.Lptr[r]]) : one);
+            T.push_back(n > 1 ? findex(m.Lent[m.Lptr[r] + 1]) : one);
             T.push_back(0u);
         }
     }
@@ -968,7 +1033,24 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
     for (int i : sorder)
         T.push_back(i < 0 ? 0xffffffffu
                           : (uint32_t)i | ((uint32_t)m.hor[i] << 16) | ((uint32_t)m.hort[i] << 24));
-    rounds(sorder, m.Vptr, m.Vent, p.oVRm, p.oVRe, 4, 0, (uint32_t)m.R);   // padding: the zero slot A[R]
+    // var entries: with every |coefficient| <= 2048 (exact in FP16, v * v exact
+    // in FP32) and A's byte offsets below 2^16, as FP16 coefficient << 16 |
+    // byte offset of A[reaction]; else reaction | v << 16
+    p.vhalf = 4 * (m.R + 1) <= 0xffff ? 1 : 0;
+    for (uint32_t e : m.Vent) {
+        const int v = (int)(int16_t)(e >> 16);
+        if (v > 2048 || v < -2048) p.vhalf = 0;
+    }
+    if (p.vhalf) {
+        std::vector<uint32_t> vent(m.Vent.size());
+        for (size_t h = 0; h < m.Vent.size(); ++h) {
+            const _Float16 hv = (_Float16)(float)(int16_t)(m.Vent[h] >> 16);
+            vent[h] = (uint32_t)__builtin_bit_cast(uint16_t, hv) << 16 | (m.Vent[h] & 0xffffu) * 4u;
+        }
+        rounds(sorder, m.Vptr, vent, p.oVRm, p.oVRe, 4, 0, (uint32_t)m.R * 4u);   // padding: A[R] = 0
+    } else {
+        rounds(sorder, m.Vptr, m.Vent, p.oVRm, p.oVRe, 4, 0, (uint32_t)m.R);       // padding: the zero slot A[R]
+    }
     rounds(sorder, FSptr, FSent, p.oFRm, p.oFRe, 4, 0, 0u);
 }
 

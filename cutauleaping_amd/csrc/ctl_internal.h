@@ -44,7 +44,7 @@ struct Model {
     std::vector<int32_t> cs;                      // SS
     int64_t row_offset = 0;
     // per-simulation feeds (read_feeds T.cu:459-567): species fs[u] is reset to
-    // fv[row][u] after every leap and SSA step; none when fs is empty
+    // fv[row][u] after every accepted leap; none when fs is empty
     std::vector<int32_t> fs;
     std::vector<uint32_t> fv;                     // fv_rows * fs.size()
     int64_t fv_rows = 0;

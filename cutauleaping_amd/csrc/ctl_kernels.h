@@ -72,13 +72,16 @@ struct KParams {
     // lane l at T[o*e + base_r + k*G + l] (base_r = G * sum of the earlier
     // lengths).  Shorter lists are padded with 0 (multiplicity / change 0);
     // var and fear round lengths are multiples of 4 (loads issue 4 at a time).
-    // Reactant entries (species | multiplicity << 16), var entries of the
-    // tau' sums and species-major fear entries (reaction | change << 16).
+    // Reactant entries (species | multiplicity << 16; with pfast the index of
+    // the reactant's factor in the group's factor table, padding 2S), var
+    // entries of the tau' sums and species-major fear entries (reaction |
+    // change << 16).
     // Var and fear rounds list species in the order T[oSPI + r*G + l]
     // (species | HOR << 16 | HORtype << 24; 0xffffffff: none), longest lists
     // first.  gsimple: no species has HOR 3 or HOR 2 of type 2, so g is 1 or 2.
     int oPRm, oPRe, oVRm, oVRe, oFRm, oFRe, oSPI, gsimple;
-    int oDR;                  // pfast: SSA dependent records {r, entry 0, entry 1, 0} (uint4, Dptr order)
+    int vhalf;                // var entries as FP16 coefficient << 16 | byte offset of A[reaction] (every |v| <= 2048)
+    int oDR;                  // pfast: SSA dependent records {r, factor index 0, factor index 1, 0} (uint4, Dptr order)
     int pfast;                // every reaction has <= 2 reactant entries of multiplicity <= 2:
                               //   the reactant stream then has exactly 2 entries per round
     const uint32_t* horw;     // hor | hort << 8
@@ -91,7 +94,7 @@ struct KParams {
     const uint32_t* x0_sims;  // [row][S] or null; row = sim / x0_div
     int c_div, x0_div;        // simulations per input row (1: one row each)
     int nfs;                  // per-simulation feeds: species fs[u] reset to
-    const int32_t* fs;        //   fv[sim / fv_div][u] after every leap and SSA step
+    const int32_t* fs;        //   fv[sim / fv_div][u] after every accepted leap
     const uint32_t* fv;
     int fv_div;
     // run
@@ -104,6 +107,8 @@ struct KParams {
     int force_ssa;
     int iters;
     int ssa_chunk;
+    int compact;              // one-lane kernel: iterations between two regroupings of a
+                              //   workgroup's slots by mode (0: never)
     // slots (global per-lane storage, [.][ns])
     int ns;
     int32_t* X;               // [2][S][ns]
@@ -429,10 +434,6 @@ __device__ __forceinline__ bool ssa_steps(const KParams& p, Lane& L, const View&
             const int v = (int)(int16_t)(e >> 16);
             X[s] = (int32_t)((uint32_t)X[s] + (uint32_t)v);
         }
-        if (p.nfs > 0) {                 // feeds (T.cu:2253-2258); their readers are in the dependency lists
-            const uint32_t* fv = p.fv + (size_t)(L.sim / p.fv_div) * p.nfs;
-            for (int u = 0; u < p.nfs; ++u) X[(size_t)CT_I32(p.fs)[u] * V.st] = (int32_t)fv[u];
-        }
         L.ssa_steps++;
         if (tn <= p.tmax) L.firings++;
         if (st + 1 < steps) {
@@ -562,6 +563,72 @@ __device__ __forceinline__ void finish(const KParams& p, Lane& L, const View& V)
     for (int s = 0; s < p.S; ++s) p.xfinal[(size_t)L.sim * p.S + s] = X[(size_t)s * V.st];
     atomicAdd(p.done, 1u);
     L.sim = -1;
+}
+
+
+// ---------------------------------------------------------------------------
+// Regrouping of a workgroup's slots by mode (one-lane kernel).  A lane runs
+// whatever slot it holds: a wavefront whose lanes hold slots in both modes
+// executes a leap and an SSA chunk one after the other, each with part of its
+// lanes idle.  Every p.compact iterations the lanes of a workgroup hand their
+// slots over through LDS so that leap-mode slots fill the first wavefronts,
+// SSA-mode slots the next ones and idle slots the last: each wavefront then
+// runs one mode.  A slot keeps its column (its state arrays, its global slot
+// index): only the lane that runs it changes, so no trajectory does.
+// ---------------------------------------------------------------------------
+constexpr int LANE_WORDS = 30;      // words of a Lane's per-slot state
+
+__device__ __forceinline__ void lane_put(const Lane& L, uint32_t* W, int col, int B) {
+    uint32_t* w = W + col;
+    int k = 0;
+    auto put32 = [&](uint32_t x) { w[(size_t)(k++) * B] = x; };
+    auto put64 = [&](uint64_t x) { put32((uint32_t)x); put32((uint32_t)(x >> 32)); };
+    put32((uint32_t)L.sim);
+    put64((uint64_t)__double_as_longlong(L.t));
+    put64((uint64_t)__double_as_longlong(L.tacc));
+    put32(L.cursor); put32((uint32_t)L.mode); put32(L.evc); put32((uint32_t)L.ssa_left); put32(L.xb);
+    put64(L.firings); put64(L.ssa_steps); put64(L.poisson); put64(L.gaps);
+    put32(L.leaps); put32(L.rejections); put32(L.switches); put32(L.absorbed);
+    put32(L.g.sim); put32(L.g.ev); put32(L.g.blk);
+    put32(L.g.b0); put32(L.g.b1); put32(L.g.b2); put32(L.g.b3); put32(L.g.pos);
+}
+
+__device__ __forceinline__ void lane_get(Lane& L, const uint32_t* W, int col, int B) {
+    const uint32_t* w = W + col;
+    int k = 0;
+    auto get32 = [&]() { return w[(size_t)(k++) * B]; };
+    auto get64 = [&]() { const uint64_t lo = get32(); return lo | ((uint64_t)get32() << 32); };
+    L.sim = (int)get32();
+    L.t = __longlong_as_double((long long)get64());
+    L.tacc = __longlong_as_double((long long)get64());
+    L.cursor = get32(); L.mode = (int)get32(); L.evc = get32(); L.ssa_left = (int)get32(); L.xb = get32();
+    L.firings = get64(); L.ssa_steps = get64(); L.poisson = get64(); L.gaps = get64();
+    L.leaps = get32(); L.rejections = get32(); L.switches = get32(); L.absorbed = get32();
+    L.g.sim = get32(); L.g.ev = get32(); L.g.blk = get32();
+    L.g.b0 = get32(); L.g.b1 = get32(); L.g.b2 = get32(); L.g.b3 = get32(); L.g.pos = get32();
+}
+
+// New column of this lane after a stable partition of the workgroup's
+// columns by key (0 leap, 1 SSA, 2 idle); cnt is LDS scratch of 3 words per
+// wavefront, perm of B words.  Every lane of the workgroup must call it.
+__device__ __forceinline__ int regroup(int key, int col, uint32_t* cnt, uint32_t* perm) {
+    const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));      // lanes below this one
+    uint32_t rank = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint64_t b = __ballot(key == k);
+        if (key == k) rank = (uint32_t)__popcll(b & lt);
+        if (lane == 0) cnt[wave * 3 + k] = (uint32_t)__popcll(b);
+    }
+    __syncthreads();
+    uint32_t pos = 0;
+    for (int k = 0; k < key; ++k)
+        for (int w = 0; w < nw; ++w) pos += cnt[w * 3 + k];
+    for (int w = 0; w < wave; ++w) pos += cnt[w * 3 + key];
+    perm[pos + rank] = (uint32_t)col;
+    __syncthreads();
+    return (int)perm[threadIdx.x];
 }
 
 }  // namespace ctl

@@ -419,14 +419,19 @@ int ctl_model_structure(const ctl_model* h, uint8_t* hor, uint8_t* hortype, int3
 // is non-zero in the LAST line of MX_feed are the fed species; rows
 // [offset, offset+n) give each simulation's values for them (tab-separated,
 // all S columns).  The reference compiles this out (DISABLE_FEED, T.cu:34),
-// so feeds are only active when a caller loads them.
+// so feeds are only active when a caller loads them.  A missing MX_feed is a
+// warning, not an error (T.cu:473-477): the model keeps no feeds.
 int ctl_model_load_feeds(ctl_model* h, const char* dir, int64_t offset, int64_t n, int verbose) {
     clear_error();
     if (!h || !dir || offset < 0) return fail(CTL_E_ARGS, "bad arguments");
     Model& m = h->m;
     std::vector<std::string> lines;
-    if (!read_lines(std::string(dir) + "/MX_feed", lines) || lines.empty())
-        return fail(CTL_E_NOFILE, "WARNING: cannot read feed quantities");
+    if (!read_lines(std::string(dir) + "/MX_feed", lines) || lines.empty()) {
+        std::printf("WARNING: cannot read feed quantities\n");
+        std::fflush(stdout);
+        m.fs.clear(); m.fv.clear(); m.fv_rows = 0;
+        return 0;
+    }
     std::vector<std::pair<const char*, const char*>> tok;
     const std::string& last = lines.back();
     split_tabs(last.data(), last.data() + last.size(), tok);

@@ -24,7 +24,6 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <mutex>
 #include <vector>
 
 #include "ctl_group.h"
@@ -37,34 +36,50 @@ namespace ctl {
 
 // One slot per lane; kLDS keeps the lane's state, trial state, propensities
 // (and per-simulation rates) in LDS for the duration of the launch, copied
-// from / to the global slot arrays at entry / exit.
+// from / to the global slot arrays at entry / exit.  With p.compact > 0 (and
+// more than one wavefront per workgroup) the workgroup regroups its slots by
+// mode every p.compact iterations (regroup): a lane then runs the slot of
+// another column, whose state it takes over through the LDS staging area
+// that follows the slot arrays; idle lanes stay in the loop until the whole
+// workgroup is idle.
 template <bool kLDS>
-__global__ __launch_bounds__(256) void tau_leap_kernel(KParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void tau_leap_kernel(KParams p) {
     extern __shared__ uint32_t smem[];
     __shared__ float recip[64];
+    __shared__ uint32_t rg_cnt[12];
     for (int i = threadIdx.x; i < 64; i += blockDim.x) recip[i] = c_recip[i];
     __syncthreads();
+    const int B = (int)blockDim.x;
+    int col = (int)threadIdx.x;
     Lane L;
-    L.slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (L.slot >= p.ns) return;
-    View V;
-    if (kLDS) {
-        const size_t B = blockDim.x;
-        int32_t* base = (int32_t*)smem + threadIdx.x;
-        V.X = base;
-        V.xbuf = (size_t)p.S * B;
-        V.A = (uint32_t*)(base + 2 * V.xbuf);
-        V.Cs = (float*)(V.A + (size_t)p.R * B);
-        V.st = B;
-    } else {
-        V.X = p.X + L.slot;
-        V.xbuf = (size_t)p.S * p.ns;
-        V.A = p.A + L.slot;
-        V.Cs = p.Cs ? p.Cs + L.slot : nullptr;
-        V.st = (size_t)p.ns;
-    }
+    L.slot = blockIdx.x * B + col;
+    if (L.slot >= p.ns) return;          // ns is a multiple of the workgroup size when regrouping
+    const bool regrouping = p.compact > 0 && B > 64;
+    const size_t lane_words = kLDS ? (size_t)2 * p.S + p.R + (p.c_bcast ? 0 : p.R) : 0;
+    uint32_t* W = smem + lane_words * B;                     // staging: LANE_WORDS x B, then perm B
+    uint32_t* perm = W + (size_t)LANE_WORDS * B;
+    auto view_of = [&](int c) {
+        View V;
+        if (kLDS) {
+            int32_t* base = (int32_t*)smem + c;
+            V.X = base;
+            V.xbuf = (size_t)p.S * B;
+            V.A = (uint32_t*)(base + 2 * V.xbuf);
+            V.Cs = (float*)(V.A + (size_t)p.R * B);
+            V.st = (size_t)B;
+        } else {
+            const int slot = blockIdx.x * B + c;
+            V.X = p.X + slot;
+            V.xbuf = (size_t)p.S * p.ns;
+            V.A = p.A + slot;
+            V.Cs = p.Cs ? p.Cs + slot : nullptr;
+            V.st = (size_t)p.ns;
+        }
+        return V;
+    };
+    View V = view_of(col);
     load_lane(p, L);
-    if (L.sim == -2) return;             // queue was exhausted for this slot
+    if (L.sim == -2 && !regrouping) return;   // queue was exhausted for this slot
     if (kLDS && L.sim >= 0) {            // resume: state from the global slot arrays
         const int32_t* G = p.X + (L.xb ? (size_t)p.S * p.ns : 0) + L.slot;
         for (int s = 0; s < p.S; ++s) V.X[(size_t)s * V.st] = G[(size_t)s * p.ns];
@@ -72,7 +87,21 @@ __global__ __launch_bounds__(256) void tau_leap_kernel(KParams p) {
         load_rates(p, V, L.sim);
     }
     for (int it = 0; it < p.iters; ++it) {
-        if (L.sim < 0 && !fetch(p, L, V)) break;
+        if (regrouping && it % p.compact == 0) {
+            // 0: leap (or about to take a simulation), 1: SSA, 2: no simulation left
+            const int key = L.sim == -2 ? 2 : (L.sim >= 0 && L.mode == MODE_SSA) || (L.sim < 0 && p.force_ssa) ? 1 : 0;
+            if (!__syncthreads_or(key != 2)) break;
+            lane_put(L, W, col, B);
+            col = regroup(key, col, rg_cnt, perm);
+            lane_get(L, W, col, B);
+            L.slot = blockIdx.x * B + col;
+            V = view_of(col);
+        }
+        if (L.sim == -2) continue;
+        if (L.sim < 0 && !fetch(p, L, V)) {
+            if (regrouping) continue;
+            break;
+        }
         if (L.mode == MODE_LEAP) {
             const int r = leap(p, L, V, recip);
             if (r == LEAP_DONE || r == LEAP_ABSORB) {
@@ -161,25 +190,70 @@ __global__ void fitness_kernel(const uint32_t* __restrict__ samples, const uint3
     fit[tid] = (1.0 / K) * (1.0 / SS) * sub;
 }
 
-// Per-group moments of the samples: one workgroup per group, one thread per
-// (sample, species) column; consecutive threads read consecutive words of
-// each simulation's sample row.
-__global__ void moments_kernel(const uint32_t* __restrict__ samples, double* __restrict__ mean,
-                               double* __restrict__ sd, int group, int KS) {
-    const int g = blockIdx.x;
-    for (int ks = threadIdx.x; ks < KS; ks += blockDim.x) {
+// Per-group moments of the samples in two passes.  Pass 1: the replicas of a
+// group are cut into min(ceil(group / MOM_CHUNK), MOM_MAX_CHUNKS) equal
+// chunks (the last one takes the remainder); a workgroup sums one chunk over a
+// tile of 256 (sample,
+// species) columns, one thread per column, so consecutive threads read
+// consecutive words of each simulation's sample row and a single large group
+// still spreads over many workgroups.  Partial sums (s1, s2) go to
+// part[(g * nchunk + c) * 2 + {0, 1}][KS].  Pass 2 adds a column's chunks in
+// chunk order and writes mean and population sd to out[0 or 1][g][KS].
+// Every partial is written before it is read, so part needs no clearing.
+constexpr int MOM_CHUNK = 256;
+constexpr int MOM_MAX_CHUNKS = 64;
+constexpr int MOM_TILE = 256;
+
+__host__ __device__ inline int moment_chunks(int group) {
+    const int c = (group + MOM_CHUNK - 1) / MOM_CHUNK;
+    return c < MOM_MAX_CHUNKS ? c : MOM_MAX_CHUNKS;
+}
+
+// Doubles of device memory the two passes need beside the samples: the
+// partial sums and the mean / sd staging.
+inline size_t moment_words(int groups, int group, int KS) {
+    return (size_t)groups * KS * 2 * (size_t)moment_chunks(group) + (size_t)2 * groups * KS;
+}
+
+__global__ void moments_partial(const uint32_t* __restrict__ samples, double* __restrict__ part, int group, int KS,
+                                int nchunk, int ntiles, size_t nblocks) {
+    const int per = group / nchunk;                     // replicas per chunk; the last chunk takes the rest
+    for (size_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const int tile = (int)(b % (size_t)ntiles);
+        const int c = (int)((b / (size_t)ntiles) % (size_t)nchunk);
+        const size_t g = b / ((size_t)ntiles * nchunk);
+        const int ks = tile * MOM_TILE + (int)threadIdx.x;
+        if (ks >= KS) continue;
+        const int r0 = c * per, r1 = c + 1 == nchunk ? group : r0 + per;
+        const uint32_t* col = samples + (g * group + r0) * (size_t)KS + ks;
         double s1 = 0.0, s2 = 0.0;
-        const uint32_t* col = samples + (size_t)g * group * KS + ks;
-        for (int r = 0; r < group; ++r) {
-            const double v = (double)col[(size_t)r * KS];
+        for (int r = r0; r < r1; ++r) {
+            const double v = (double)col[(size_t)(r - r0) * KS];
             s1 += v;
             s2 += v * v;
+        }
+        double* o = part + ((g * nchunk + c) * 2) * (size_t)KS + ks;
+        o[0] = s1;
+        o[KS] = s2;
+    }
+}
+
+__global__ void moments_final(const double* __restrict__ part, double* __restrict__ out, int groups, int group,
+                              int KS, int nchunk) {
+    const size_t total = (size_t)groups * KS;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t g = idx / KS, ks = idx % KS;
+        double s1 = 0.0, s2 = 0.0;
+        for (int c = 0; c < nchunk; ++c) {
+            const double* o = part + ((g * nchunk + c) * 2) * (size_t)KS + ks;
+            s1 += o[0];
+            s2 += o[KS];
         }
         const double m = s1 / group;
         double var = s2 / group - m * m;
         if (var < 0.0) var = 0.0;
-        if (mean) mean[(size_t)g * KS + ks] = m;
-        if (sd) sd[(size_t)g * KS + ks] = sqrt(var);
+        out[idx] = m;
+        out[total + idx] = sqrt(var);
     }
 }
 
@@ -200,6 +274,7 @@ struct ctl_sim {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int n = 0, ns = 0, block = 256, grid = 0;
     int lanes = 1;                   // lanes per simulation (1: tau_leap_kernel, else group_kernel<lanes>)
+    int compact = 0;                 // one-lane kernel: iterations between regroupings by mode (0: off)
     bool lds = false;
     size_t smem = 0;
     KParams kp{};
@@ -257,6 +332,40 @@ void release(ctl_sim* s) {
 }
 
 }  // namespace
+
+// Argument checks shared by the moment entry points.
+static int moments_check(const ctl_sim* s, int32_t group, size_t count) {
+    if (!s) return ctl::fail(CTL_E_ARGS, "null sim");
+    if (!s->ran) return ctl::fail(CTL_E_STATE, "ctl_sim_run has not completed");
+    if (group <= 0 || s->n % group != 0) return ctl::fail(CTL_E_ARGS, "group must divide the simulation count");
+    if (count != (size_t)(s->n / group) * s->kp.K * s->kp.SS) return ctl::fail(CTL_E_ARGS, "count must be (n/group)*K*SS");
+    return 0;
+}
+
+// The two moment passes on the sim's stream, with `work` (device memory of
+// moment_words doubles) for the partial sums and the mean / sd staging, then
+// the copies to the caller's buffers.  No allocation.
+static int moments_run(ctl_sim* s, int32_t group, double* mean, double* sd, size_t count, double* work) {
+    HIPCHK(hipSetDevice(s->device), "hipSetDevice");
+    const int KS = s->kp.K * s->kp.SS;
+    const int groups = s->n / group;
+    const int nchunk = moment_chunks(group);
+    const int ntiles = (KS + MOM_TILE - 1) / MOM_TILE;
+    const size_t nblocks = (size_t)groups * nchunk * ntiles;
+    double* part = work;
+    double* out = work + (size_t)groups * KS * 2 * nchunk;
+    const unsigned g1 = (unsigned)std::min<size_t>(nblocks, 65536);
+    hipLaunchKernelGGL(moments_partial, dim3(g1), dim3(MOM_TILE), 0, s->stream, s->kp.storage, part, group, KS,
+                       nchunk, ntiles, nblocks);
+    HIPCHK(hipGetLastError(), "moments_partial launch");
+    const unsigned g2 = (unsigned)std::min<size_t>((count + 255) / 256, 65536);
+    hipLaunchKernelGGL(moments_final, dim3(g2), dim3(256), 0, s->stream, part, out, groups, group, KS, nchunk);
+    HIPCHK(hipGetLastError(), "moments_final launch");
+    HIPCHK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    if (mean) HIPCHK(hipMemcpy(mean, out, sizeof(double) * count, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    if (sd) HIPCHK(hipMemcpy(sd, out + count, sizeof(double) * count, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return 0;
+}
 
 int ctl::sim_dims(const ctl_sim* s, int* n, int* K, int* SS, const double** ts) {
     if (!s) return ctl::fail(CTL_E_ARGS, "null sim");
@@ -359,16 +468,33 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
         s->ns = (int)(groups * G);
         s->grid = (int)(groups / gpb);
     } else {
+        // regrouping slots by mode needs several wavefronts per workgroup
+        const int compact = o->compact_interval > 0 ? o->compact_interval : 0;    // measured slower by default
         s->lds = o->block_size < 0 ? false : lds_auto;
-        s->block = o->block_size > 0 ? o->block_size : (s->lds ? 64 : 256);
+        auto smem_of = [&](bool lds, int block) {
+            return (lds ? lane_words * 4 * block : 0) +
+                   (compact > 0 && block > 64 ? (size_t)(LANE_WORDS + 1) * 4 * block : 0);
+        };
+        auto block_of = [&](bool lds) {
+            if (o->block_size > 0) return o->block_size;
+            if (compact <= 0) return lds ? 64 : 256;
+            return smem_of(lds, 256) <= 65536 ? 256 : 128;
+        };
+        s->block = block_of(s->lds);
         if (s->block % 64 != 0 || s->block > 256) return bail(ctl::fail(CTL_E_ARGS, "block_size must be 64..256, multiple of 64"));
-        s->smem = s->lds ? lane_words * 4 * s->block : 0;
-        if (s->smem > 65536) { s->lds = false; s->smem = 0; }
+        s->smem = smem_of(s->lds, s->block);
+        if (s->smem > 65536) {
+            s->lds = false;
+            s->block = block_of(false);
+            s->smem = smem_of(false, s->block);
+        }
+        s->compact = compact > 0 && s->block > 64 ? compact : 0;
         const void* kfn = s->lds ? (const void*)tau_leap_kernel<true> : (const void*)tau_leap_kernel<false>;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, s->block, s->smem);
         if (e != hipSuccess || per_cu <= 0) per_cu = 1;
         int ns = o->slots > 0 ? o->slots : prop.multiProcessorCount * per_cu * s->block;
         if (ns > n) ns = ((n + s->block - 1) / s->block) * s->block;
+        if (s->compact) ns = ((ns + s->block - 1) / s->block) * s->block;   // whole workgroups
         s->ns = ns;
         s->grid = (ns + s->block - 1) / s->block;
     }
@@ -395,9 +521,6 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
             list.clear();
             for (uint32_t h = m.Fptr[j]; h < m.Fptr[j + 1]; ++h)
                 for (uint32_t r : readers[m.Fent[h] & 0xffffu])
-                    if (!mark[r]) { mark[r] = 1; list.push_back(r); }
-            for (int32_t fsp : m.fs)     // feeds reset these species after every SSA step
-                for (uint32_t r : readers[fsp])
                     if (!mark[r]) { mark[r] = 1; list.push_back(r); }
             std::sort(list.begin(), list.end());
             for (uint32_t r : list) { mark[r] = 0; Dent.push_back(r); }
@@ -488,6 +611,7 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     p.ssa_factor = o->ssa_factor;
     p.force_ssa = o->force_ssa ? 1 : 0;
     p.ssa_chunk = o->ssa_chunk > 0 ? o->ssa_chunk : 10;
+    p.compact = s->compact;
     p.iters = o->iters_per_launch > 0 ? o->iters_per_launch : (G > 1 ? 65536 : 4096);
     p.ns = ns;
     p.Cs = nullptr;
@@ -547,7 +671,7 @@ int ctl_sim_run(ctl_sim* s) {
                 if (s->lds)
                     hipLaunchKernelGGL(tau_leap_kernel<true>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p);
                 else
-                    hipLaunchKernelGGL(tau_leap_kernel<false>, dim3(s->grid), dim3(s->block), 0, s->stream, p);
+                    hipLaunchKernelGGL(tau_leap_kernel<false>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p);
                 break;
             case 2: hipLaunchKernelGGL(group_kernel<2>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
             case 4: hipLaunchKernelGGL(group_kernel<4>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
@@ -673,23 +797,14 @@ int ctl_fitness(const ctl_sim* sc, double* out, size_t count) {
 int ctl_sim_moments(const ctl_sim* sc, int32_t group, double* mean, double* sd, size_t count) {
     ctl::clear_error();
     ctl_sim* s = const_cast<ctl_sim*>(sc);
-    if (!s) return ctl::fail(CTL_E_ARGS, "null sim");
-    if (!s->ran) return ctl::fail(CTL_E_STATE, "ctl_sim_run has not completed");
-    if (group <= 0 || s->n % group != 0) return ctl::fail(CTL_E_ARGS, "group must divide the simulation count");
-    const int KS = s->kp.K * s->kp.SS;
-    const int groups = s->n / group;
-    if (count != (size_t)groups * KS) return ctl::fail(CTL_E_ARGS, "count must be (n/group)*K*SS");
-    HIPCHK(hipSetDevice(s->device), "hipSetDevice");
+    int rc = moments_check(s, group, count);
+    if (rc) return rc;
+    const size_t words = moment_words(s->n / group, group, s->kp.K * s->kp.SS);
     double* dev = nullptr;
-    HIPCHK(hipMalloc(&dev, sizeof(double) * 2 * count), "hipMalloc");
-    hipLaunchKernelGGL(moments_kernel, dim3(groups), dim3(256), 0, s->stream, s->kp.storage, dev, dev + count, group, KS);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    if (e == hipSuccess && mean) e = hipMemcpy(mean, dev, sizeof(double) * count, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && sd) e = hipMemcpy(sd, dev + count, sizeof(double) * count, hipMemcpyDeviceToHost);
+    HIPCHK(hipMalloc(&dev, sizeof(double) * words), "hipMalloc");
+    rc = moments_run(s, group, mean, sd, count, dev);
     (void)hipFree(dev);
-    if (e != hipSuccess) return hip_fail(e, "moments_kernel");
-    return 0;
+    return rc;
 }
 
 int ctl_write_samples_binary(const ctl_sim* s, const char* path) {

@@ -140,12 +140,28 @@ def sample_times(K, t_end):
     return np.array([t_end * (i + 1) / K for i in range(K)], dtype=np.float64)
 
 
-def _batched_one(args):
-    i, seed, t_end, check, observed = args
-    rng = np.random.default_rng([seed, i, 0xB7])
+def _batched_size(rng):
     N = int(np.floor(10.0 ** rng.uniform(1.0, np.log10(500.0))))
     M = int(np.floor(10.0 ** rng.uniform(1.0, 3.0)))
     replicas = int(np.floor(10.0 ** rng.uniform(np.log10(64.0), np.log10(4096.0))))
+    return N, M, replicas
+
+
+def batched_sizes(count=8192, seed=2):
+    """(N, M, replicas) of models 0..count-1 of batched_models(count, seed),
+    without drawing their networks: an int array [count, 3]."""
+    return np.array([_batched_size(np.random.default_rng([seed, i, 0xB7])) for i in range(count)], np.int64)
+
+
+def batched_model(i, seed=2, t_end=1.0, check=True, observed=10):
+    """Model i of batched_models(..., seed): the same dict as its list entry."""
+    return _batched_one((i, seed, t_end, check, observed))
+
+
+def _batched_one(args):
+    i, seed, t_end, check, observed = args
+    rng = np.random.default_rng([seed, i, 0xB7])
+    N, M, replicas = _batched_size(rng)
     net = mass_action(N, M, seed=seed * 100003 + i, t_end=t_end, check=check)
     net["replicas"] = replicas
     net["cs"] = np.sort(rng.choice(N, size=min(observed, N), replace=False)).astype(np.int32)
@@ -158,19 +174,29 @@ def batched_models(count=8192, seed=2, t_end=1.0, check=True, observed=10, proce
     floor(10^U(log10 64, log10 4096)) replicas, then its network as
     mass_action(N, M, seed * 100003 + i); `observed` sampled species per model.
     Returns a list of dicts (left, right, x0, c, attempt, replicas, cs).
-    processes > 1 draws the models in forked workers (BLAS limited to one
-    thread first); log(msg) reports progress every 128 models."""
+    processes > 1 draws the models in spawned workers whose BLAS / OpenMP
+    pools are one thread each (forked workers inherit the parent's thread
+    pools and stall); log(msg) reports progress every 128 models."""
     jobs = [(i, seed, t_end, check, observed) for i in range(count)]
     out = []
     if processes > 1:
         import multiprocessing as mp
-        from threadpoolctl import threadpool_limits
-        with threadpool_limits(limits=1):
-            with mp.get_context("fork").Pool(processes) as pool:
-                for net in pool.imap(_batched_one, jobs, chunksize=4):
+        import os
+        keys = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")
+        saved = {k: os.environ.get(k) for k in keys}
+        os.environ.update({k: "1" for k in keys})
+        try:
+            with mp.get_context("spawn").Pool(processes) as pool:
+                for net in pool.imap(_batched_one, jobs, chunksize=8):
                     out.append(net)
                     if log and len(out) % 128 == 0:
                         log(f"# {len(out)}/{count} models drawn")
+        finally:
+            for k, v in saved.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
         return out
     for j in jobs:
         out.append(_batched_one(j))

@@ -83,6 +83,10 @@ typedef struct {
     int32_t  lanes_per_sim;    /* 1: one lane runs one simulation; 2..64 (power of two): that many
                                   lanes of a wavefront cooperate on one simulation whose state
                                   stays in LDS (the group algorithm); 0: chosen by the library */
+    int32_t  compact_interval; /* one lane per simulation: every this many iterations a workgroup
+                                  of several wavefronts regroups its simulations so that leap-mode
+                                  and SSA-mode ones run in different wavefronts (<= 0: never, the
+                                  default); trajectories do not depend on it */
 } ctl_opts;
 
 typedef struct {
@@ -152,11 +156,14 @@ int ctl_model_structure(const ctl_model* m, uint8_t* hor, uint8_t* hortype,
 int ctl_model_load_fitness(ctl_model* m, const char* dir, int32_t threads, int32_t blocks, int verbose);
 
 /* Per-simulation feeds, read_feeds (T.cu:459-567): the species with a
- * non-zero value in the last line of MX_feed are reset, after every leap and
- * every SSA step (T.cu:2072-2079, 2253-2258, 2434-2440), to the value in the
- * simulation's row (rows [offset, offset+n); n <= 0: all from offset).  The
+ * non-zero value in the last line of MX_feed are reset, after every accepted
+ * leap (T.cu:2072-2079), to the value in the simulation's row (rows
+ * [offset, offset+n); n <= 0: all from offset).  SSA steps apply no feed, as
+ * in the reference's USE_FEAR_SSA build (T.cu:38, 2240-2244, 2420-2424).  The
  * reference compiles feeds out by default (DISABLE_FEED, T.cu:34): a model
- * has none unless one of these is called.  Rows follow the x0_rows rule. */
+ * has none unless one of these is called.  A missing MX_feed prints the
+ * reference's warning and leaves the model without feeds (T.cu:473-477).
+ * Rows follow the x0_rows rule. */
 int ctl_model_load_feeds(ctl_model* m, const char* dir, int64_t offset, int64_t n, int verbose);
 int ctl_model_set_feeds(ctl_model* m, const int32_t* species, int32_t nfs, const uint32_t* values, int64_t rows);
 
@@ -213,14 +220,18 @@ int ctl_write_samples_binary(const ctl_sim* s, const char* path);
 void ctl_sim_destroy(ctl_sim* s);
 
 /* ---- batched models ------------------------------------------------------ */
-/* Many independent models of uneven size and replica count in one persistent
- * launch.  No counterpart in the reference, which runs one model per process
- * (a batch replaces N invocations of kernel.cu's main with the same argv
- * except the input folder).  Model i runs replicas[i] simulations; its
+/* Many independent models of uneven size and replica count in one run of
+ * persistent kernels: the models that share a number of lanes per simulation
+ * form a class with its own kernel and stream, and the classes run
+ * concurrently.  No counterpart in the reference, which runs one model per
+ * process (a batch replaces N invocations of kernel.cu's main with the same
+ * argv except the input folder).  Model i runs replicas[i] simulations; its
  * Philox stream indices are offset + base_i + replica, base_i being the sum
  * of the replica counts of the models before it (ctl_batch_stream_base).
  * threads / blocks of the options are ignored; every other field applies to
- * all models.  The models must outlive nothing: they are copied. */
+ * all models (block_size to the one-lane class only).  The models must
+ * outlive nothing: they are copied.  ctl_stats.kernel_ms of a batch is the
+ * elapsed time of its launch sequence (the classes overlap). */
 typedef struct ctl_batch ctl_batch;
 int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, int32_t nmodels,
                      const ctl_opts* o, ctl_batch** out);
@@ -230,6 +241,13 @@ int ctl_batch_samples(const ctl_batch* b, int32_t model, uint32_t* out, size_t c
 int ctl_batch_sim_stats(const ctl_batch* b, int32_t model, ctl_sim_stat* out, size_t count);
 int ctl_batch_stats(const ctl_batch* b, ctl_stats* out);
 int ctl_batch_stream_base(const ctl_batch* b, int32_t model, int64_t* base);
+/* Lanes per simulation model `model` runs with: lanes_per_sim of the options
+ * when it is non-zero, otherwise the library's choice for that model (one
+ * lane for small working sets, else about 16 reactions per lane, widened
+ * until a wavefront's share of LDS fits).  Models of one value form a kernel
+ * class; ctl_stats.lanes_per_sim of a batch is that value when every model
+ * shares it and 0 when the classes differ. */
+int ctl_batch_model_lanes(const ctl_batch* b, int32_t model, int32_t* lanes);
 void ctl_batch_destroy(ctl_batch* b);
 
 /* ---- surrogate MLP ------------------------------------------------------- */

@@ -494,9 +494,14 @@ static int apply_leap(simctx* s, int first, int pick, float tauf, double taud, i
 
 static void swap_state(simctx* s) { int32_t* tmp = s->X; s->X = s->X1; s->X1 = tmp; }
 
-/* Per-simulation feeds (read_feeds T.cu:459-567, applied after every accepted
- * leap, T.cu:2072-2079, and after every SSA step, T.cu:2253-2258 and
- * 2434-2440): each feed species is reset to the simulation's feed value. */
+/* Per-simulation feeds (read_feeds T.cu:459-567): after every accepted leap
+ * each feed species is reset to the simulation's feed value (T.cu:2072-2079).
+ * SSA steps apply no feed: the reference's SSA kernels update with the fear
+ * matrix under USE_FEAR_SSA (T.cu:38, 2240-2244, 2420-2424), whose branch has
+ * no feed block (the blocks at T.cu:2253-2258 and 2434-2440 sit in its #else).
+ * The feed row is the simulation's own row (the reference reads row tid,
+ * T.cu:2077, a per-block index; the loader stores rows per simulation,
+ * T.cu:529). */
 static void apply_feeds(simctx* s) {
     for (int u = 0; u < s->J->nfs; ++u) s->X[s->J->fs[u]] = (int32_t)s->fv[u];
 }
@@ -637,7 +642,6 @@ static void ssa_f32(simctx* s) {
         ssa_samples(s, tn);
         for (int h = m->Fptr[j]; h < m->Fptr[j + 1]; ++h)
             s->X[m->Fs[h]] = (int32_t)((uint32_t)s->X[m->Fs[h]] + (uint32_t)m->Fv[h]);
-        apply_feeds(s);
         s->st.ssa_steps++;
         if (tn <= s->tmax) s->st.firings++;
     }
@@ -763,7 +767,6 @@ static void ssa_f64(simctx* s) {
         ssa_samples(s, tn);
         for (int h = m->Fptr[j]; h < m->Fptr[j + 1]; ++h)
             s->X[m->Fs[h]] = (int32_t)((uint32_t)s->X[m->Fs[h]] + (uint32_t)m->Fv[h]);
-        apply_feeds(s);
         s->st.ssa_steps++;
         if (tn <= s->tmax) s->st.firings++;
     }
@@ -1003,7 +1006,6 @@ static void g_ssa(grpctx* q) {
         ssa_samples(s, tn);
         for (int h = m->Fptr[j]; h < m->Fptr[j + 1]; ++h)
             s->X[m->Fs[h]] = (int32_t)((uint32_t)s->X[m->Fs[h]] + (uint32_t)m->Fv[h]);
-        apply_feeds(s);
         s->st.ssa_steps++;
         if (tn <= s->tmax) s->st.firings++;
     }

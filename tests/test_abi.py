@@ -140,16 +140,17 @@ def test_cli_usage_exit_code(ctl):
     assert p.returncode == 255 and "please specify" in p.stdout
 
 
-def test_feed_loader(ctl, tmp_path):
+def test_feed_loader(ctl, tmp_path, capfd):
     """MX_feed: the species non-zero in the last line are fed; rows from the
-    offset; width and row errors mirror MX_0's."""
+    offset; width and row errors mirror MX_0's.  A missing MX_feed is the
+    reference's warning, not an error (2phase_n-tau-leaping.cu:473-477)."""
     import shutil
     folder = tmp_path / "m"
     shutil.copytree(os.path.join(GOLDEN, "schlogl"), folder)
     m = ctl.Model.from_folder(str(folder), 0, 4)
-    with pytest.raises(ctl.CtlError) as e:
-        m.load_feeds(str(folder), 0, 4)
-    assert e.value.code == ctl.CTL_E_NOFILE
+    capfd.readouterr()
+    m.load_feeds(str(folder), 0, 4)
+    assert "WARNING: cannot read feed quantities" in capfd.readouterr().out
     (folder / "MX_feed").write_text("".join(f"0\t{10 + i}\t0\n" for i in range(4)))
     m.load_feeds(str(folder), 0, 4)
     with pytest.raises(ctl.CtlError) as e:

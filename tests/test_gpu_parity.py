@@ -303,3 +303,64 @@ def test_replica_group_rows_bit_exact(gpu, oracle_lib):
         mean, sd = s.moments(32)
     ref = out.reshape(16, 32, 20, 10).astype(float)
     assert np.allclose(mean, ref.mean(axis=1)) and np.allclose(sd, ref.std(axis=1), atol=1e-6)
+
+
+def test_cli_fitness_to_pref_allfit(gpu, oracle_lib, tmp_path):
+    """fitness = 1 (kernel.cu:406-420): the fitness of each (group, thread)
+    goes to ./pref_allfit in the working directory, one line per group of
+    p_rep blocks, fixed notation with 20 decimals, tab-terminated values;
+    the values equal the closed form over the oracle's samples."""
+    print("case: CLI fitness=1 (pref_allfit)", flush=True)
+    env = dict(os.environ, CTL_SEED="4242")
+    fold = tmp_path / "fit"
+    subprocess.run(["cp", "-r", os.path.join(GOLDEN, "schlogl"), str(fold)], check=True)
+    (fold / "p_rep").write_text("2\n")
+    (fold / "ts_rep").write_text("1\n")
+    (fold / "ts_numtgt").write_text("3\n")
+    (fold / "tts_vector").write_text("128\n")
+    (fold / "ts_matrix").write_text("".join(f"{t}\t100000\t300\t200000\n" for t in SCH_TS))
+    work = tmp_path / "cwd"
+    work.mkdir()
+    p = subprocess.run([gpu.CLI_PATH, str(fold), "32", "4", "0", "0", str(tmp_path / "o"), "fit", "1", "0"],
+                       env=env, capture_output=True, text=True, timeout=300, cwd=str(work))
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert " * Dumping fitness to file." in p.stdout
+    lines = (work / "pref_allfit").read_text().split("\n")
+    assert lines[-1] == "" and len(lines) == 3            # 4 blocks / p_rep 2 = 2 groups
+    ref, _ = oracle_lib.run(SCH["left"], SCH["right"], SCH["x0"], SCH["c"], SCH_TS, locked=SCH["feed"], n=128,
+                            seed=4242)
+    tgt = np.array([100000, 300, 200000], float)
+    for g, line in enumerate(lines[:2]):
+        assert line.endswith("\t")
+        fields = line.split("\t")[:-1]
+        assert len(fields) == 32
+        assert all("." in f and len(f.split(".")[1]) == 20 for f in fields)
+        for th in range(32):
+            m2 = (ref[(2 * g) * 32 + th].astype(float) + ref[(2 * g + 1) * 32 + th].astype(float)) / 2
+            exp = np.sum(np.abs(tgt - m2) / tgt) / (100 * 3)
+            assert abs(float(fields[th]) - exp) <= 1e-5 * exp + 1e-7, (g, th)
+
+
+@pytest.mark.parametrize("compact,block_size,force,iters,slots", [
+    (1, 0, False, 7, 300),
+    (3, 0, True, 11, 0),
+    (2, -1, False, 9, 0),
+    (8, 0, False, 0, 0),
+    (-1, 0, False, 0, 0),
+])
+def test_one_lane_regrouping_bit_exact(gpu, oracle_lib, compact, block_size, force, iters, slots):
+    """The one-lane kernel regroups a workgroup's simulations by mode (leap /
+    SSA) every `compact` iterations, a lane taking over another slot's state;
+    with state in LDS (Schlogl) and in HBM (block_size -1), small iteration
+    budgets and slot counts that park simulations across launches, every
+    sample and counter equals the oracle's (which knows no scheduling)."""
+    print(f"case: one-lane regrouping compact={compact} block_size={block_size} force={force} iters={iters}",
+          flush=True)
+    m = gpu.Model.from_arrays(SCH["left"], SCH["right"], SCH["x0"], SCH["c"], SCH_TS, feed=SCH["feed"])
+    with gpu.Sim(m, 64, 16, seed=606, force_ssa=force, lanes_per_sim=1, compact_interval=compact,
+                 block_size=block_size, iters_per_launch=iters, slots=slots) as s:
+        s.run()
+        out, st, xf = oracle_lib.run(SCH["left"], SCH["right"], SCH["x0"], SCH["c"], SCH_TS, locked=SCH["feed"],
+                                     n=1024, seed=606, force_ssa=force, final=True)
+        _compare(s, out, st, xf)
+    m.close()
