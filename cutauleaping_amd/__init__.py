@@ -53,7 +53,7 @@ EXPORTS = (
     "ctl_batch_sim_stats", "ctl_batch_stats", "ctl_batch_stream_base", "ctl_batch_destroy",
     "ctl_mlp_create", "ctl_mlp_num_params", "ctl_mlp_params", "ctl_mlp_forward", "ctl_mlp_forward_device",
     "ctl_mlp_gradient", "ctl_mlp_set_data", "ctl_mlp_train", "ctl_mlp_destroy",
-    "ctl_model_load_feeds", "ctl_model_set_feeds", "ctl_batch_model_lanes",
+    "ctl_model_load_feeds", "ctl_model_set_feeds", "ctl_sim_moments_noalloc", "ctl_batch_model_lanes",
 )
 
 
@@ -144,6 +144,7 @@ def lib():
         L.ctl_write_dynamics.argtypes = [vp, C.c_char_p, C.c_int]
         L.ctl_fitness.argtypes = [vp, vp, C.c_size_t]
         L.ctl_sim_moments.argtypes = [vp, i32, vp, vp, C.c_size_t]
+        L.ctl_sim_moments_noalloc.argtypes = [vp, i32, vp, vp, C.c_size_t]
         L.ctl_write_samples_binary.argtypes = [vp, C.c_char_p]
         L.ctl_batch_create.argtypes = [vp, vp, i32, C.POINTER(ctl_opts), C.POINTER(vp)]
         L.ctl_batch_run.argtypes = [vp]
@@ -323,13 +324,14 @@ class Sim:
         _check(lib().ctl_fitness(self._h, out.ctypes.data, out.size))
         return out
 
-    def moments(self, group: int):
+    def moments(self, group: int, noalloc: bool = False):
         """Per-group mean and standard deviation, [n/group, K, SS] each (on the
-        device)."""
+        device); noalloc=True uses the allocation-free entry point."""
         g = self.n // group
         mean = np.zeros((g, self.K, self.SS), np.float64)
         sd = np.zeros_like(mean)
-        _check(lib().ctl_sim_moments(self._h, group, mean.ctypes.data, sd.ctypes.data, mean.size))
+        fn = lib().ctl_sim_moments_noalloc if noalloc else lib().ctl_sim_moments
+        _check(fn(self._h, group, mean.ctypes.data, sd.ctypes.data, mean.size))
         return mean, sd
 
     def write_samples_binary(self, path: str) -> None:

@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ctl_group.h"
@@ -257,6 +258,14 @@ __global__ void moments_final(const double* __restrict__ part, double* __restric
     }
 }
 
+// Device scratch of the allocation-free moment entry point
+// (ctl_sim_moments_noalloc): the partial sums and the mean / sd staging of
+// the two moment passes.  Each GPU the library runs on holds its own copy
+// until the process ends.  Host code never uses this name as an address: it
+// takes the per-device address from hipGetSymbolAddress (scratch_setup).
+constexpr size_t SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) double scratch[SCRATCH_BYTES / sizeof(double)];
+
 }  // namespace ctl
 
 // ===========================================================================
@@ -329,6 +338,29 @@ void release(ctl_sim* s) {
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->ev0 = s->ev1 = nullptr;
     s->stream = nullptr;
+}
+
+// Device address of `scratch` on each device, looked up once per device
+// with that device current (the first ctl_sim_create on it).
+constexpr int MAX_DEVICES = 64;
+void* g_scratch[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+int scratch_setup(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return ctl::fail(CTL_E_DEVICE, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return 0;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch[device] = ptr;
+    return 0;
+}
+
+void* scratch_ptr(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch[device];
 }
 
 }  // namespace
@@ -419,6 +451,7 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     auto bail = [&](int rc) { release(s); delete s; return rc; };
     hipError_t e = hipSetDevice(s->device);
     if (e != hipSuccess) return bail(hip_fail(e, "hipSetDevice"));
+    if (int rc0 = scratch_setup(s->device)) return bail(rc0);
     e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
     if ((e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess)
@@ -805,6 +838,23 @@ int ctl_sim_moments(const ctl_sim* sc, int32_t group, double* mean, double* sd, 
     rc = moments_run(s, group, mean, sd, count, dev);
     (void)hipFree(dev);
     return rc;
+}
+
+int ctl_sim_moments_noalloc(const ctl_sim* sc, int32_t group, double* mean, double* sd, size_t count) {
+    ctl::clear_error();
+    ctl_sim* s = const_cast<ctl_sim*>(sc);
+    int rc = moments_check(s, group, count);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * moment_words(s->n / group, group, s->kp.K * s->kp.SS);
+    if (bytes > SCRATCH_BYTES) {
+        char b[160];
+        std::snprintf(b, sizeof b, "moments need %zu bytes of device scratch; the allocation-free entry point has %zu",
+                      bytes, SCRATCH_BYTES);
+        return ctl::fail(CTL_E_LIMIT, b);
+    }
+    void* ptr = scratch_ptr(s->device);
+    if (!ptr) return ctl::fail(CTL_E_STATE, "device scratch not set up");
+    return moments_run(s, group, mean, sd, count, static_cast<double*>(ptr));
 }
 
 int ctl_write_samples_binary(const ctl_sim* s, const char* path) {

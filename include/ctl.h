@@ -212,6 +212,18 @@ int ctl_fitness(const ctl_sim* s, double* out, size_t count);
  * multiple of group.  Either output pointer may be NULL. */
 int ctl_sim_moments(const ctl_sim* s, int32_t group, double* mean, double* sd, size_t count);
 
+/* Allocation-free variant of ctl_sim_moments, same arguments and the same
+ * results bit for bit.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc /
+ * hipFree: the partial sums and the staging of the two reduction passes live
+ * in a 32 MiB static device array, one per GPU, set up by the first
+ * ctl_sim_create on that GPU.  Fails with CTL_E_LIMIT when the moments need
+ * more (about 16 * (n/group) * K * SS * (ceil(group/256), at most 64, + 1)
+ * bytes).  Every call on a GPU shares that GPU's array: calls that overlap on
+ * one GPU (from several threads) race and must be serialised by the caller. */
+int ctl_sim_moments_noalloc(const ctl_sim* s, int32_t group, double* mean, double* sd, size_t count);
+
 /* Binary dump of the samples: a 32-byte header (magic "CTLS", version 1,
  * n, K, SS as uint32, 12 reserved bytes) then n*K*SS uint32 in
  * [simulation][sample][sampled species] order. */
