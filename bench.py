@@ -80,17 +80,22 @@ def main():
     ap.add_argument("--ssa-chunk", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per simulation (0: library's choice)")
     ap.add_argument("--compact", type=int, default=0,
-                    help="one lane per simulation: iterations between regroupings by mode (<=0: off, the default)")
+                    help="iterations between regroupings of simulations by mode (0: library default, <0: off)")
     ap.add_argument("--c0-sims", type=int, default=1024, help="c0: simulations (the configuration runs 1024)")
-    ap.add_argument("--workload", default="c1", choices=("c0", "c1", "c2", "c3", "c4"),
+    ap.add_argument("--workload", default="c1", choices=("c0", "c1", "c2", "c3", "c4", "mlp"),
                     help="c1: the headline (default); c0: Schlogl; c2: batched uneven models; "
-                         "c3: surrogate-screened sweep; c4: enzyme cascade (adaptive and force-SSA)")
+                         "c3: surrogate-screened sweep; c4: enzyme cascade (adaptive and force-SSA); "
+                         "mlp: the surrogate network's kernels alone")
+    ap.add_argument("--mlp-rows", type=int, default=65536, help="mlp: rows (parameter sets) per pass")
     ap.add_argument("--models", type=int, default=8192, help="c2: models in the batch")
+    ap.add_argument("--no-tune", action="store_true", help="c2: the library's size rule instead of the autotuner")
     ap.add_argument("--c3-train", type=int, default=65536)
     ap.add_argument("--c3-test", type=int, default=4096)
     ap.add_argument("--c3-replicas", type=int, default=256)
     ap.add_argument("--c3-judge", type=int, default=512)
     ap.add_argument("--c3-epochs", type=int, default=60)
+    ap.add_argument("--phase-clock", action="store_true",
+                    help="c1: run the profiling build (libctl_phases.so) and report the group kernel's time per phase")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device once (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -106,14 +111,16 @@ def main():
             args.steps = 1
         if "--warmup" not in sys.argv:
             args.warmup = 0
-        {"c0": bench_c0, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[args.workload](args)
+        {"c0": bench_c0, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "mlp": bench_mlp}[args.workload](args)
         return
 
     import cutauleaping_amd as ctl
     from cutauleaping_amd import networks
 
+    if args.phase_clock:                 # the profiling build: same kernels plus the phase clock
+        ctl.LIB_PATH = os.path.join(os.path.dirname(ctl.LIB_PATH), "libctl_phases.so")
     if not os.path.exists(ctl.LIB_PATH):
-        raise SystemExit("bench.py: libctl.so is not built (python __graft_entry__.py build)")
+        raise SystemExit(f"bench.py: {ctl.LIB_PATH} is not built (python __graft_entry__.py build)")
     t0 = time.time()
     net = networks.mass_action(100, 200, seed=args.seed, t_end=args.t_end)
     ts = networks.sample_times(args.samples, args.t_end)
@@ -135,7 +142,8 @@ def main():
         if k not in sims:
             sims[k] = ctl.Sim(model, threads, n // threads, seed=0xC0FFEE, offset=k * n,
                               block_size=args.block_size, slots=args.slots, iters_per_launch=args.iters,
-                              ssa_chunk=args.ssa_chunk, lanes_per_sim=args.lanes)
+                              ssa_chunk=args.ssa_chunk, lanes_per_sim=args.lanes,
+                              compact_interval=args.compact)
         return sims[k]
 
     for k in range(min(nslices, args.steps + args.warmup)):
@@ -148,6 +156,9 @@ def main():
     launches = 0
     work = 0.0
     timed = 0
+    clock = np.zeros(16, np.uint64)
+    if args.phase_clock:
+        ctl._check(ctl.lib().ctl_phase_clock(0, clock.ctypes.data, 16, 1))     # zero the totals
     t_start = time.perf_counter()
     for k in range(args.steps):
         sim = slice_sim(k)
@@ -204,11 +215,23 @@ def main():
                                           "poisson", "gaps", "absorbed", "lds_state", "slots", "block_size",
                                           "lanes_per_sim")},
     }
+    if args.phase_clock:
+        ctl._check(ctl.lib().ctl_phase_clock(0, clock.ctypes.data, 16, 0))
+        tot = float(clock.sum())
+        out["phases"] = {"unit": "share of the wavefronts' shader-clock ticks", "total_ticks": int(tot),
+                         "ticks_per_leap": tot / max(last["leaps"] * args.steps, 1),
+                         "share": {name: float(clock[i]) / max(tot, 1.0) for i, name in enumerate(PHASES)}}
     for x in sims.values():
         x.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(net, ts, args.cpu_sims)
     print(json.dumps(out), flush=True)
+
+
+# phases of the group kernel's phase clock (ctl_phase_clock), in its order
+PHASES = ("propensities", "tau1", "switch_tau2_pick", "counts_gaps", "poisson_queue", "update", "sampling",
+          "ssa_open", "ssa_select", "ssa_update", "ssa_dependents", "before_leap", "before_ssa", "loop_fetch_finish",
+          "regroup", "park")
 
 
 def copy_probe(nbytes):
@@ -277,7 +300,7 @@ def cpu_baseline(net, ts, nsims, locked=None, cs=None, force_ssa=False):
     dt = time.perf_counter() - t0
     return {"value": float(st["firings"].sum()) / dt, "unit": "firings/s", "cores": threads,
             "kind": "port", "sample": f"{nsims} simulations of the same workload (full t_end), "
-                                      f"{dt:.1f}s, {int(st['firings'].sum())} firings",
+                                      f"{dt:.3f}s, {int(st['firings'].sum())} firings",
             "simulations_per_s": nsims / dt, "host": host}
 
 
@@ -344,11 +367,22 @@ def bench_c2(args):
     print(f"# {len(nets)} models generated in {time.time() - t0:.1f}s on {procs} processes", file=sys.stderr,
           flush=True)
     ts = networks.sample_times(100, t_end)
-    models = [ctl.Model.from_arrays(n["left"], n["right"], n["x0"], n["c"], ts, cs=n["cs"]) for n in nets]
     reps = [n["replicas"] for n in nets]
+    tuned = None
+    if args.lanes == 0 and not args.no_tune:
+        from cutauleaping_amd import autotune
+        t0 = time.time()
+        for n in nets:
+            n["ts"] = ts
+        tuned = autotune.tune_batch(nets, reps, log=lambda m: print(m, file=sys.stderr, flush=True))
+        tuned["seconds"] = time.time() - t0
+        print(f"# autotuned lanes per class in {tuned['seconds']:.1f}s: "
+              f"{ {g: c['chosen'] for g, c in tuned['classes'].items()} }", file=sys.stderr, flush=True)
+    models = [ctl.Model.from_arrays(n["left"], n["right"], n["x0"], n["c"], ts, cs=n["cs"]) for n in nets]
     fir = 0
     ms = 0.0
-    with ctl.Batch(models, reps, seed=0xC0FFEE, lanes_per_sim=args.lanes, verbose=True) as b:
+    with ctl.Batch(models, reps, seed=0xC0FFEE, lanes_per_sim=args.lanes, verbose=True,
+                   lanes=None if tuned is None else tuned["lanes"]) as b:
         lanes = np.array([b.model_lanes(i) for i in range(len(models))])
         classes = {int(g): {"models": int((lanes == g).sum()), "simulations": int(np.asarray(reps)[lanes == g].sum())}
                    for g in np.unique(lanes)}
@@ -369,7 +403,11 @@ def bench_c2(args):
           {"workload": f"{len(nets)} mass-action models, N 10..500, M 10..1000, {sims} simulations "
                        f"(64..4096 replicas each), 100 samples to t_end={t_end:g}, one batch of concurrent "
                        f"per-class persistent kernels", "models": len(nets), "simulations": sims, "t_end": t_end,
-           "classes": classes},
+           "classes": classes,
+           "autotune": None if tuned is None else
+           {"seconds": tuned["seconds"], "classes": {str(g): {"probed_model": c["model"], "chosen": c["chosen"],
+                                                              "scores": {str(k): v for k, v in c["scores"].items()}}
+                                                     for g, c in tuned["classes"].items()}}},
           simulations_per_s=sims * args.steps / (ms / 1e3),
           counters={k: st[k] for k in ("leaps", "ssa_steps", "switches", "poisson", "gaps", "launches",
                                        "slots", "lanes_per_sim")},
@@ -424,9 +462,67 @@ def bench_c3(args):
           r["screened_sweep_s"] * 1e3, "bf16",
           {"workload": f"mass-action N=50 M=100 seed=5, 32 uncertain rates (log-uniform, 2 decades), 10 observed "
                        f"species x 50 times to t_end=5; {args.c3_train} training sets x {args.c3_replicas} "
-                       f"replicas; {args.c3_test} held-out sets judged with {r['judge_replicas']} replicas; "
+                       f"replicas; threshold chosen on {r['n_val']} validation sets; {args.c3_test} held-out test sets "
+                       f"judged with {r['judge_replicas']} replicas; "
                        f"MLP 32-256-256-256-1000, 5 members", "n_train": args.c3_train, "n_test": args.c3_test},
           study=r)
+
+
+def bench_mlp(args):
+    """The surrogate's network alone (32-256-256-256-1000, BF16 operands, FP32
+    accumulation on MFMA): one member's forward pass over --mlp-rows parameter
+    sets on device buffers, timed with HIP events on the library's stream;
+    then training steps of one batch of --mlp-rows rows (gather, forward,
+    loss, backward, Adam), timed on the host around whole epochs.
+    Algorithmic FLOPs: 2 * rows * sum(w_l * w_(l+1)) forward, twice that for
+    the backward GEMMs (input errors and weight gradients; the first layer's
+    input error is not computed, so slightly less)."""
+    import ctypes as C
+    import cutauleaping_amd as ctl
+    widths = [32, 256, 256, 256, 1000]
+    rows = args.mlp_rows
+    macs = sum(a * b for a, b in zip(widths[:-1], widths[1:]))
+    f_fwd = 2.0 * rows * macs
+    f_bwd = 2.0 * rows * (2 * macs - widths[0] * widths[1])
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, (rows, widths[0])).astype(np.float32)
+    t = rng.standard_normal((rows, widths[-1])).astype(np.float32)
+    net = ctl.MLP(widths, seed=1)
+    xd, yd = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(xd), x.nbytes) or hip.hipMalloc(C.byref(yd), rows * widths[-1] * 4):
+        raise SystemExit("bench.py: hipMalloc failed")
+    try:
+        if hip.hipMemcpy(xd, x.ctypes.data, x.nbytes, 1):
+            raise SystemExit("bench.py: hipMemcpy failed")
+        steps = max(args.steps, 5)
+        for _ in range(max(args.warmup, 2)):
+            net.forward_device(xd.value, rows, yd.value)
+        ms = [net.forward_device(xd.value, rows, yd.value) for _ in range(steps)]
+    finally:
+        hip.hipFree(xd)
+        hip.hipFree(yd)
+    fwd_ms = float(np.mean(ms))
+    fwd_tf = f_fwd / (fwd_ms * 1e-3) / 1e12
+    net.set_data(x, t)
+    net.train(2, rows, 1e-4, seed=1)                     # warm-up epochs (one step each)
+    t0 = time.perf_counter()
+    net.train(steps, rows, 1e-4, seed=2)
+    step_ms = (time.perf_counter() - t0) * 1e3 / steps
+    net.close()
+    train_tf = (f_fwd + f_bwd) / (step_ms * 1e-3) / 1e12
+    peak = 2500.0
+    _line("surrogate MLP forward throughput (BF16 MFMA, FP32 accumulate)", fwd_tf, "TFLOP/s", steps, fwd_ms,
+          "bf16", {"workload": f"MLP 32-256-256-256-1000, one member, {rows} rows per pass", "rows": rows},
+          roofline={"bound": "mfma", "achieved": fwd_tf, "peak": peak, "unit": "TFLOP/s", "frac": fwd_tf / peak,
+                    "traffic": None},
+          flops_per_pass=f_fwd,
+          train={"step_ms": step_ms, "tflops": train_tf, "frac": train_tf / peak,
+                 "flops_per_step": f_fwd + f_bwd,
+                 "note": "host-timed whole epochs of one step: includes the epoch's index upload and sync"})
 
 
 if __name__ == "__main__":

@@ -54,6 +54,7 @@ EXPORTS = (
     "ctl_mlp_create", "ctl_mlp_num_params", "ctl_mlp_params", "ctl_mlp_forward", "ctl_mlp_forward_device",
     "ctl_mlp_gradient", "ctl_mlp_set_data", "ctl_mlp_train", "ctl_mlp_destroy",
     "ctl_model_load_feeds", "ctl_model_set_feeds", "ctl_sim_moments_noalloc", "ctl_batch_model_lanes",
+    "ctl_batch_create_lanes", "ctl_phase_clock",
 )
 
 
@@ -147,6 +148,8 @@ def lib():
         L.ctl_sim_moments_noalloc.argtypes = [vp, i32, vp, vp, C.c_size_t]
         L.ctl_write_samples_binary.argtypes = [vp, C.c_char_p]
         L.ctl_batch_create.argtypes = [vp, vp, i32, C.POINTER(ctl_opts), C.POINTER(vp)]
+        L.ctl_batch_create_lanes.argtypes = [vp, vp, vp, i32, C.POINTER(ctl_opts), C.POINTER(vp)]
+        L.ctl_phase_clock.argtypes = [i32, vp, i32, i32]
         L.ctl_batch_run.argtypes = [vp]
         L.ctl_batch_samples.argtypes = [vp, i32, vp, C.c_size_t]
         L.ctl_batch_sim_stats.argtypes = [vp, i32, vp, C.c_size_t]
@@ -360,7 +363,9 @@ class Batch:
 
     def __init__(self, models, replicas, *, gpu=0, offset=0, force_ssa=False, seed=0x5EEDC0DE, nc=10,
                  eps=0.03, ssa_steps=100, ssa_factor=10.0, block_size=0, slots=0, iters_per_launch=0,
-                 ssa_chunk=0, lanes_per_sim=0, verbose=False):
+                 ssa_chunk=0, lanes_per_sim=0, lanes=None, verbose=False):
+        """lanes: optional lanes per simulation for each model (0 = the
+        library's choice), e.g. from autotune.tune_batch."""
         o = default_opts()
         o.verbose = int(bool(verbose))
         o.gpu, o.offset, o.force_ssa, o.seed = gpu, offset, int(bool(force_ssa)), seed
@@ -372,8 +377,15 @@ class Batch:
         self.info = [m.info() for m in self.models]
         handles = (C.c_void_p * len(self.models))(*[m._h.value for m in self.models])
         self._h = C.c_void_p()
-        _check(lib().ctl_batch_create(handles, self.replicas.ctypes.data, len(self.models), C.byref(o),
-                                      C.byref(self._h)))
+        if lanes is None:
+            _check(lib().ctl_batch_create(handles, self.replicas.ctypes.data, len(self.models), C.byref(o),
+                                          C.byref(self._h)))
+        else:
+            lv = np.ascontiguousarray(lanes, dtype=np.int32)
+            if lv.shape != (len(self.models),):
+                raise ValueError("lanes must give one value per model")
+            _check(lib().ctl_batch_create_lanes(handles, self.replicas.ctypes.data, lv.ctypes.data,
+                                                len(self.models), C.byref(o), C.byref(self._h)))
 
     def run(self) -> "Batch":
         _check(lib().ctl_batch_run(self._h))

@@ -347,6 +347,11 @@ extern "C" {
 
 int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, int32_t nmodels, const ctl_opts* o,
                      ctl_batch** out) {
+    return ctl_batch_create_lanes(models, replicas, nullptr, nmodels, o, out);
+}
+
+int ctl_batch_create_lanes(const ctl_model* const* models, const int64_t* replicas, const int32_t* lanes_in,
+                           int32_t nmodels, const ctl_opts* o, ctl_batch** out) {
     ctl::clear_error();
     if (!models || !replicas || !o || !out || nmodels <= 0) return ctl::fail(CTL_E_ARGS, "bad arguments");
     *out = nullptr;
@@ -360,7 +365,7 @@ int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, in
             (m.c_rows > 1 && m.c_rows < replicas[i] && replicas[i] % m.c_rows))
             return ctl::fail(CTL_E_ROWS, "a model's input rows must be 1, at least its replicas, or divide them");
         total += replicas[i];
-        const int G = o->lanes_per_sim > 0 ? o->lanes_per_sim : auto_lanes(m);
+        const int G = lanes_in && lanes_in[i] > 0 ? lanes_in[i] : o->lanes_per_sim > 0 ? o->lanes_per_sim : auto_lanes(m);
         if (G < 1 || G > 64 || (G & (G - 1)))
             return ctl::fail(CTL_E_ARGS, "lanes_per_sim must be 0 or a power of two up to 64");
         if (G > 1 && (size_t)(64 / G) * ctl::group_lds_words(m.S, m.R, G) * 4 > 65536)

@@ -24,6 +24,9 @@ __global__ __launch_bounds__(64) void gbatch_kernel(GBatchParams bp) {
     const int slot = (int)blockIdx.x;
     if (slot >= bp.ngroups) return;
     GLane L;
+    L.tacc = 0.0;
+    L.ssa_left = 0;
+    L.ssa_mode = 0;
     L.slot = slot;
     L.gg.tag = 0u;
     L.lg.tag = 1u + (uint32_t)l;
@@ -143,6 +146,9 @@ void gclass_kernel(GClassParams bp) {
     if (wave >= bp.nwaves) return;
     uint32_t* gbase = smem + (size_t)gib * (size_t)bp.gwords;
     GLane L;
+    L.tacc = 0.0;
+    L.ssa_left = 0;
+    L.ssa_mode = 0;
     L.slot = wave * (64 / G) + gib;
     L.gg.tag = 0u;
     L.lg.tag = 1u + (uint32_t)l;

@@ -56,6 +56,61 @@ __device__ __forceinline__ void gsync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ---------------------------------------------------------------------------
+// Phase clock, compiled only into the profiling build of the library
+// (CTL_PHASE_CLOCK=1: libctl_phases.so, ctl_phase_clock): the time the
+// wavefronts spend in each phase of the group kernel, from s_memtime read at
+// phase boundaries and summed over the launch's wavefronts.  The segment
+// between two consecutive marks of a wavefront is added to the phase of the
+// later mark by the wavefront's lowest active lane (the last mark's time is
+// kept per wavefront, so the two sides of a divergent branch count once
+// each).  The normal build compiles the marks to nothing.
+// ---------------------------------------------------------------------------
+enum : int {
+    PH_PROPS = 0, PH_TAU1, PH_PICK, PH_COUNTS, PH_POISSON, PH_UPDATE, PH_SAMPLE,
+    PH_SSA_OPEN, PH_SSA_SELECT, PH_SSA_UPDATE, PH_SSA_DEPS,
+    PH_PRE_LEAP, PH_PRE_SSA, PH_LOOP, PH_REGROUP, PH_PARK, PH_COUNT
+};
+#if CTL_PHASE_CLOCK
+static __device__ unsigned long long phase_total[PH_COUNT];
+static __shared__ unsigned long long phase_row[4][PH_COUNT + 1];    // per wavefront; last: time of the last mark
+__device__ __forceinline__ bool phase_leader() {
+    return (int)(threadIdx.x & 63u) == __builtin_ffsll((long long)__ballot(1)) - 1;
+}
+__device__ __forceinline__ void phase_start() {
+    if (phase_leader()) {
+        unsigned long long* r = phase_row[threadIdx.x >> 6];
+        for (int k = 0; k < PH_COUNT; ++k) r[k] = 0ull;
+        r[PH_COUNT] = __builtin_amdgcn_s_memtime();
+    }
+}
+__device__ __forceinline__ void phase_mark(int k) {
+    const unsigned long long t = __builtin_amdgcn_s_memtime();
+    if (phase_leader()) {
+        unsigned long long* r = phase_row[threadIdx.x >> 6];
+        r[k] += t - r[PH_COUNT];
+        r[PH_COUNT] = t;
+    }
+}
+__device__ __forceinline__ void phase_flush() {
+    if (phase_leader()) {
+        unsigned long long* r = phase_row[threadIdx.x >> 6];
+        for (int k = 0; k < PH_COUNT; ++k)
+            if (r[k]) {
+                atomicAdd(&phase_total[k], r[k]);
+                r[k] = 0ull;
+            }
+    }
+}
+#define CTL_PHASE_START() phase_start()
+#define CTL_PHASE(k) phase_mark(k)
+#define CTL_PHASE_FLUSH() phase_flush()
+#else
+#define CTL_PHASE_START() ((void)0)
+#define CTL_PHASE(k) ((void)0)
+#define CTL_PHASE_FLUSH() ((void)0)
+#endif
+
 // A kernel parameter read at its point of use: the compiler may not hoist a
 // test of it out of the main loop, where it would hold a 64-bit lane mask in
 // scalar registers for the whole kernel (and spill others).
@@ -211,6 +266,9 @@ struct GLane {
     const uint32_t* fvrow;   // the simulation's feed values (when the model has feeds)
     Rng gg;                  // group stream (word 3 = 0)
     Rng lg;                  // this lane's stream (word 3 = 1 + lane)
+    double tacc;             // clock advance of the SSA burst in progress
+    int ssa_left;            // steps left in the SSA burst in progress (0: none)
+    int ssa_mode;            // 1 once a leap attempt switched, until its burst has run
 };
 
 struct GView {
@@ -490,6 +548,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
     PC = gscan<G>(PC, l);
     const float a0 = gbcast_last<G>(P, l);
     const float a0c = gbcast_last<G>(PC, l);
+    CTL_PHASE(PH_PROPS);
     if (a0 == 0.0f) { gabsorb<G>(p, L, X, l); return LEAP_ABSORB; }
     // tau' (T.cu:1860-1922): the one-lane kernel's per-species values, min over lanes
     float tau1 = 3.40282347e+38f;
@@ -555,6 +614,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         if (tmp < tau1) tau1 = tmp;
     }
     tau1 = gmin<G>(tau1);
+    CTL_PHASE(PH_TAU1);
     if (tau1 < p.ssa_factor / a0) { L.switches++; return LEAP_SWITCH; }      // T.cu:1935-1941
     uint32_t e = L.evc++;
     rng_event(L.gg, e);
@@ -573,6 +633,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         Q = gscan<G>(Q, l);
         pick = gselect<G>(p, V.A, Qown, Q, w, l, 2);
     }
+    CTL_PHASE(PH_PICK);
     uint64_t fired = 0;
     for (int attempt = 0;; ++attempt) {                                        // T.cu:2011-2050
         if (attempt > 0) { e = L.evc++; L.rejections++; }
@@ -625,6 +686,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
             if (need) Q[atomicAdd(V.qn, 1u)] = (uint16_t)j;
         }
         gsync();
+        CTL_PHASE(PH_COUNTS);
         const uint32_t nq = *V.qn;
         uint64_t f = fk;
         for (uint32_t qi = (uint32_t)l; qi < nq; qi += G) {
@@ -639,6 +701,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
             L.poisson++;
         }
         gsync();
+        CTL_PHASE(PH_POISSON);
         bool bad = false;
         const uint32_t* F = p.T + p.oFRe + l;
         for (int r = 0; r < nrs; ++r) {
@@ -668,6 +731,7 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         fired = gsum64<G>(f);
         const bool any_bad = gballot<G>(bad) != 0;
         gsync();
+        CTL_PHASE(PH_UPDATE);
         if (!any_bad) break;
         tau = tau * 0.5f;
     }
@@ -703,27 +767,36 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
     L.t = L.t + (double)tau;
     L.firings += fired;
     L.leaps++;
+    CTL_PHASE(PH_SAMPLE);
     return LEAP_OK;
 }
 
-// One SSA burst (EventualSSASteps T.cu:2099-2273 / LightweightSSA
-// T.cu:2280-2454); after the fired reaction only its dependent propensities
-// are recomputed (same values as recomputing all).  Returns true on absorption.
+// Up to nmax steps of an SSA burst (EventualSSASteps T.cu:2099-2273 /
+// LightweightSSA T.cu:2280-2454), opening the burst when none is in progress:
+// a burst of p.ssa_steps steps may run in several calls, its clock advance,
+// propensities and factor table kept meanwhile (L.tacc, L.ssa_left, the
+// group's LDS), so the steps and their draws are those of one call.  After
+// the fired reaction only its dependent propensities are recomputed (same
+// values as recomputing all).  Returns true on absorption.
 template <int G>
-__device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V, int l) {
+__device__ __forceinline__ bool gssa_run(const KParams& p, GLane& L, const GView& V, int l, int nmax) {
     int32_t* X = V.Xb + (L.xb ? p.S : 0);
-    rng_event(L.gg, L.evc++);
-    double tacc = 0.0;
-    if (kuse(p.pfast)) {
-        gfactors<G>(p, X, V.F, l);
-        gsync();
-    }
-    {
+    if (L.ssa_left == 0) {
+        rng_event(L.gg, L.evc++);
+        L.tacc = 0.0;
+        L.ssa_left = p.ssa_steps;
+        if (kuse(p.pfast)) {
+            gfactors<G>(p, X, V.F, l);
+            gsync();
+        }
         float P, PC;
         gprops<G>(p, X, V.F, L.crate, V.A, l, P, PC);
+        gsync();
     }
-    gsync();
-    for (int step = 0; step < p.ssa_steps; ++step) {
+    CTL_PHASE(PH_SSA_OPEN);
+    double tacc = L.tacc;
+    const int n = nmax < L.ssa_left ? nmax : L.ssa_left;
+    for (int step = 0; step < n; ++step) {
         float P = 0.0f;
         const int nrp = group_rpad(p.R, G) / G;        // rounds over A with its zero padding
         for (int r0 = 0; r0 < nrp; r0 += 4) {          // padding adds an exact 0
@@ -736,11 +809,12 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
         const float Pown = P;
         P = gscan<G>(P, l);
         const float a0 = gbcast_last<G>(P, l);
-        if (a0 == 0.0f) { gabsorb<G>(p, L, X, l); return true; }
+        if (a0 == 0.0f) { gabsorb<G>(p, L, X, l); L.ssa_left = 0; return true; }
         const float r1 = rng_oc(L.gg) * a0;
         const int j = gselect<G>(p, V.A, Pown, P, r1, l, 0);
-        if (j < 0) { gabsorb<G>(p, L, X, l); return true; }    // unreachable for finite a0
+        if (j < 0) { gabsorb<G>(p, L, X, l); L.ssa_left = 0; return true; }    // unreachable for finite a0
         const float dt = exp_rv(L.gg) / a0;
+        CTL_PHASE(PH_SSA_SELECT);
         tacc = tacc + (double)dt;
         const double tn = L.t + tacc;
         uint32_t i = L.cursor;
@@ -764,6 +838,7 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
             if (fac) gfactor(V.F, p.S, s, x, (uint32_t)p.nc);   // a species occurs once per reaction
         }
         gsync();
+        CTL_PHASE(PH_SSA_UPDATE);
         L.ssa_steps++;
         if (tn <= p.tmax) L.firings++;
         const uint32_t d0 = p.T[p.oDp + j], d1 = p.T[p.oDp + j + 1];
@@ -788,9 +863,22 @@ __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V,
             }
         }
         gsync();
+        CTL_PHASE(PH_SSA_DEPS);
     }
-    L.t = L.t + tacc;
+    L.ssa_left -= n;
+    L.tacc = tacc;
+    if (L.ssa_left == 0) {
+        L.t = L.t + tacc;
+        L.tacc = 0.0;
+    }
     return false;
+}
+
+// One whole SSA burst.
+template <int G>
+__device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V, int l) {
+    L.ssa_left = 0;
+    return gssa_run<G>(p, L, V, l, p.ssa_steps);
 }
 
 __device__ __forceinline__ const float* grates(const KParams& p, int sim) {
@@ -806,6 +894,9 @@ __device__ __forceinline__ bool gfetch(const KParams& p, GLane& L, const GView& 
     if (q >= (uint32_t)p.n) { L.sim = -2; return false; }
     L.sim = (int)q;
     L.t = 0.0;
+    L.tacc = 0.0;
+    L.ssa_left = 0;
+    L.ssa_mode = kuse(p.force_ssa);
     L.cursor = 0;
     L.evc = 0;
     L.xb = 0;
@@ -851,33 +942,152 @@ __device__ __forceinline__ void gfinish(const KParams& p, GLane& L, const GView&
 // Longest a switched group waits for a partner burst (leap iterations).
 constexpr int GBURST_WAIT = 24;
 
+// ---------------------------------------------------------------------------
+// Regrouping groups by mode.  The groups of a wavefront run in lockstep: while
+// one group runs SSA steps the wavefront's leaping groups wait, and the
+// other way round.  With p.compact > 0 a workgroup's group slots ("regions":
+// a simulation's LDS state and its global slot) are reassigned to the
+// workgroup's wavefronts every p.compact iterations, leap-mode regions first,
+// SSA-mode ones next, regions without a simulation last, so that each
+// wavefront runs one mode.  SSA bursts then run in chunks of p.ssa_chunk steps
+// per iteration, comparable to a leap.  A region's simulation keeps its LDS
+// state; its registers travel
+// through the slot arrays in global memory (the parking format).  No event of
+// any simulation changes: only which lanes run it.
+// ---------------------------------------------------------------------------
+
+// Writes the group's registers to the slot arrays of its region's slot s0.
+template <int G>
+__device__ __forceinline__ void gstage_put(const KParams& p, const GLane& L, int s0, int l) {
+    p.st->poisson[(size_t)s0 * G + l] = L.poisson;
+    p.st->gaps[(size_t)s0 * G + l] = L.gaps;
+    if (l == 0) {
+        p.st->sim[s0] = L.sim;
+        p.st->t[s0] = L.t;
+        p.st->tacc[s0] = L.tacc;
+        p.st->cursor[s0] = L.cursor;
+        p.st->ssa_left[s0] = L.ssa_left;
+        p.st->mode[s0] = L.ssa_mode;
+        p.st->evc[s0] = L.evc;
+        p.st->xb[s0] = L.xb;
+        p.st->firings[s0] = L.firings;
+        p.st->ssa_steps[s0] = L.ssa_steps;
+        p.st->leaps[s0] = L.leaps;
+        p.st->rejections[s0] = L.rejections;
+        p.st->switches[s0] = L.switches;
+        p.st->ev[s0] = L.gg.ev;
+        p.st->blk[s0] = L.gg.blk;
+        p.st->pos[s0] = L.gg.pos;
+    }
+}
+
+// Takes over the simulation of slot s0: its registers, rate and feed rows and
+// group stream, whose partly consumed block is regenerated.
+template <int G>
+__device__ __forceinline__ void gstage_get(const KParams& p, GLane& L, int s0, int l) {
+    L.sim = p.st->sim[s0];
+    L.poisson = p.st->poisson[(size_t)s0 * G + l];
+    L.gaps = p.st->gaps[(size_t)s0 * G + l];
+    L.absorbed = 0;
+    if (L.sim < 0) return;
+    L.t = p.st->t[s0];
+    L.tacc = p.st->tacc[s0];
+    L.cursor = p.st->cursor[s0];
+    L.ssa_left = p.st->ssa_left[s0];
+    L.ssa_mode = p.st->mode[s0];
+    L.evc = p.st->evc[s0];
+    L.xb = p.st->xb[s0];
+    L.firings = p.st->firings[s0];
+    L.ssa_steps = p.st->ssa_steps[s0];
+    L.leaps = p.st->leaps[s0];
+    L.rejections = p.st->rejections[s0];
+    L.switches = p.st->switches[s0];
+    L.gg.ev = p.st->ev[s0];
+    L.gg.blk = p.st->blk[s0];
+    L.gg.pos = p.st->pos[s0];
+    L.gg.sim = L.lg.sim = p.sim_base + (uint32_t)L.sim;
+    L.crate = grates(p, L.sim);
+    L.fvrow = p.nfs > 0 ? p.fv + (size_t)(L.sim / p.fv_div) * p.nfs : nullptr;
+    if (L.gg.pos < 4u) {                 // the block in use: regenerate it, drop the words consumed
+        philox_block(L.gg.blk - 1u, L.gg.ev, L.gg.sim, L.gg.tag, L.gg.k0, L.gg.k1, L.gg.b0, L.gg.b1, L.gg.b2,
+                     L.gg.b3);
+        for (uint32_t i = 0; i < L.gg.pos; ++i) {
+            L.gg.b0 = L.gg.b1;
+            L.gg.b1 = L.gg.b2;
+            L.gg.b2 = L.gg.b3;
+        }
+    }
+}
+
+// New region of this group after a stable partition of the workgroup's
+// regions by key (0 leap, 1 SSA, 2 none); cnt: 3 words per wavefront, perm:
+// one word per region.  Every lane of the workgroup must call it.
+template <int G>
+__device__ __forceinline__ int gregroup(int key, int reg, uint32_t* cnt, uint32_t* perm, int l) {
+    const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t rank = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint64_t b = __ballot(l == 0 && key == k);      // one bit per group (its lane 0)
+        if (key == k) rank = (uint32_t)__popcll(b & lt);
+        if (lane == 0) cnt[wave * 3 + k] = (uint32_t)__popcll(b);
+    }
+    __syncthreads();
+    uint32_t pos = 0;
+    for (int k = 0; k < key; ++k)
+        for (int w = 0; w < nw; ++w) pos += cnt[w * 3 + k];
+    for (int w = 0; w < wave; ++w) pos += cnt[w * 3 + key];
+    if (l == 0) perm[pos + rank] = (uint32_t)reg;
+    __syncthreads();
+    return (int)perm[threadIdx.x / G];
+}
+
 // Persistent group kernel: p.ns lanes = p.ns / G group slots, each pulling
-// simulations from the queue; a launch runs up to p.iters leap attempts (each
-// with its SSA burst when it switched) per group, then parks the group's
-// state in the slot arrays for the next launch.
+// simulations from the queue; a launch runs up to p.iters iterations per
+// group (a leap attempt, or SSA steps), then parks the group's state in the
+// slot arrays for the next launch.  Without regrouping (RG false: G = 64, or
+// p.compact = 0) a group that switched to SSA holds its whole burst until a second
+// group of the wavefront switches too (or it has waited GBURST_WAIT
+// iterations, or every group still running is waiting), so that bursts run
+// side by side.  Either way each simulation's events and draws are its own;
+// only the order in which the lanes serve the simulations moves.
 // G = 8, 16 and 32 are compiled for four waves per SIMD (128 VGPRs, a few
 // bytes of scratch): measured +9 % at G = 16 and +4 % at G = 32 on the
 // 100x200 network, +24 % at G = 8 on the 50x100 sweep network.  G = 4
 // groups are bounded by LDS instead, and G = 64 would spill heavily.
-template <int G>
+template <int G, bool RG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G == 16 || G == 32 ? 4 : 1))) void group_kernel(KParams p) {
     extern __shared__ uint32_t smem[];
     __shared__ float recip[64];
+    __shared__ uint32_t rg_cnt[12];
+    __shared__ uint32_t rg_perm[256 / G];
     for (int i = threadIdx.x; i < 64; i += blockDim.x) recip[i] = c_recip[i];
     __syncthreads();
     const int l = (int)(threadIdx.x & (unsigned)(G - 1));
-    const int gib = (int)(threadIdx.x / G);
+    const int nreg = (int)(blockDim.x / G);
+    int gib = (int)(threadIdx.x / G);
+    const size_t gw = group_lds_words(p.S, p.R, G);
+    constexpr bool rg = RG && G < 64;     // the host launches RG only with p.compact > 0 and several wavefronts
     GLane L;
-    L.slot = (int)blockIdx.x * (int)(blockDim.x / G) + gib;
-    if (L.slot * G >= p.ns) return;
-    const GView V = group_view(smem + (size_t)gib * group_lds_words(p.S, p.R, G), p.S, p.R, G);
+    L.slot = (int)blockIdx.x * nreg + gib;
+    CTL_PHASE_START();
+    if (L.slot * G >= p.ns) return;                  // whole workgroups when regrouping
+    GView V = group_view(smem + (size_t)gib * gw, p.S, p.R, G);
     gzero_pad<G>(V, p.R, l);
     L.gg.k0 = L.lg.k0 = p.key0;
     L.gg.k1 = L.lg.k1 = p.key1;
     L.gg.tag = 0u;
     L.lg.tag = 1u + (uint32_t)l;
+    L.tacc = 0.0;
+    L.ssa_left = 0;
+    L.ssa_mode = 0;
     L.sim = p.st->sim[L.slot];
-    if (L.sim == -2) return;
+    if (L.sim == -2 && !rg) {
+        CTL_PHASE(PH_PARK);
+        CTL_PHASE_FLUSH();
+        return;
+    }
     if (L.sim >= 0) {                    // resume a parked simulation
         const int s0 = L.slot;
         L.t = p.st->t[s0];
@@ -890,6 +1100,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G
         L.rejections = p.st->rejections[s0];
         L.switches = p.st->switches[s0];
         L.absorbed = 0;
+        L.ssa_mode = kuse(p.force_ssa);
         L.poisson = p.st->poisson[(size_t)s0 * G + l];
         L.gaps = p.st->gaps[(size_t)s0 * G + l];
         L.gg.sim = L.lg.sim = p.sim_base + (uint32_t)L.sim;
@@ -899,20 +1110,58 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G
         L.fvrow = p.nfs > 0 ? p.fv + (size_t)(L.sim / p.fv_div) * p.nfs : nullptr;
         gsync();
     }
-    // A group that switched to SSA holds its burst until a second group of the
-    // wavefront switches too (or it has waited GBURST_WAIT iterations, or every
-    // group still running is waiting): the bursts then run side by side instead
-    // of one group's burst idling the others.  Each simulation's trajectory is
-    // unchanged; only the order in which the wavefront serves its groups moves.
     bool pend = false;
     int waited = 0;
     for (int it = 0; it < p.iters; ++it) {
+        CTL_PHASE(PH_LOOP);
+        if (rg) {
+            if (it % p.compact == 0) {
+                // 0: leap (or about to take a simulation), 1: SSA, 2: no simulation left
+                const int key = L.sim == -2 ? 2 : (L.sim >= 0 ? L.ssa_mode : kuse(p.force_ssa));
+                if (!__syncthreads_or(key != 2)) break;
+                // a group whose region stays on its lanes keeps its registers;
+                // the others hand their state over through the slot arrays
+                const int ng = gregroup<G>(key, gib, rg_cnt, rg_perm, l);
+                const bool move = ng != gib;
+                if (move) gstage_put<G>(p, L, L.slot, l);
+                __syncthreads();
+                if (move) {
+                    gib = ng;
+                    L.slot = (int)blockIdx.x * nreg + gib;
+                    gstage_get<G>(p, L, L.slot, l);
+                    V = group_view(smem + (size_t)gib * gw, p.S, p.R, G);
+                }
+                CTL_PHASE(PH_REGROUP);
+            }
+            if (L.sim == -2) continue;
+            if (L.sim < 0 && !gfetch<G>(p, L, V, l)) continue;
+            if (L.ssa_mode) {
+                CTL_PHASE(PH_PRE_SSA);
+                if (kuse(p.force_ssa) && L.ssa_left == 0 && L.t >= p.tmax) {
+                    gfinish<G>(p, L, V, l);
+                } else if (gssa_run<G>(p, L, V, l, p.ssa_chunk)) {
+                    gfinish<G>(p, L, V, l);
+                } else if (L.ssa_left == 0 && !kuse(p.force_ssa)) {
+                    L.ssa_mode = 0;
+                }
+            } else {
+                CTL_PHASE(PH_PRE_LEAP);
+                const int r = gleap<G>(p, L, V, l, recip);
+                if (r == LEAP_DONE || r == LEAP_ABSORB) {
+                    gfinish<G>(p, L, V, l);
+                } else if (r == LEAP_SWITCH) {
+                    L.ssa_mode = 1;
+                }
+            }
+            continue;
+        }
         if (L.sim < 0 && !gfetch<G>(p, L, V, l)) break;
         if (kuse(p.force_ssa)) {
             if (L.t >= p.tmax || gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);
             continue;
         }
         if (!pend) {
+            CTL_PHASE(PH_PRE_LEAP);
             const int r = gleap<G>(p, L, V, l, recip);
             if (r == LEAP_DONE || r == LEAP_ABSORB) {
                 gfinish<G>(p, L, V, l);
@@ -931,10 +1180,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G
         }
         if (go && pend) {
             pend = false;
+            CTL_PHASE(PH_PRE_SSA);
             if (gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);
         }
     }
-    if (pend && gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);   // before parking
+    // before parking: the burst of a switched simulation runs to its end (a
+    // parked simulation resumes with a leap attempt, or with a new burst in
+    // force-SSA mode)
+    if (rg) {
+        if (L.sim >= 0 && (L.ssa_left > 0 || (L.ssa_mode && !kuse(p.force_ssa))) &&
+            gssa_run<G>(p, L, V, l, L.ssa_left > 0 ? L.ssa_left : p.ssa_steps))
+            gfinish<G>(p, L, V, l);
+    } else if (pend && gssa<G>(p, L, V, l)) {
+        gfinish<G>(p, L, V, l);
+    }
     const int s0 = L.slot;
     if (L.sim >= 0) {                    // park: state to the slot arrays
         const int32_t* X = V.Xb + (L.xb ? p.S : 0);
@@ -953,6 +1212,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G
         }
     }
     if (l == 0) p.st->sim[s0] = L.sim;
+    CTL_PHASE(PH_PARK);
+    CTL_PHASE_FLUSH();
 }
 
 // Host: appends the lane-interleaved round streams (KParams::oPRm ..) of a

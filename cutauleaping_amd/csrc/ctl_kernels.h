@@ -107,8 +107,8 @@ struct KParams {
     int force_ssa;
     int iters;
     int ssa_chunk;
-    int compact;              // one-lane kernel: iterations between two regroupings of a
-                              //   workgroup's slots by mode (0: never)
+    int compact;              // iterations between two regroupings of a workgroup's
+                              //   simulations by mode (0: never)
     // slots (global per-lane storage, [.][ns])
     int ns;
     int32_t* X;               // [2][S][ns]

@@ -10,13 +10,19 @@
 // FP32; the GEMM operands (weights, activations, back-propagated errors) are
 // BF16 (round to nearest even); every product accumulates in FP32.
 //
-// GEMM: one wavefront computes a 32x32 tile of C with
-// v_mfma_f32_32x32x8_bf16_1k; a 256-lane workgroup covers 64x64.  Operand
-// element (i, k) of A is A[i*ai + k*ak], (k, j) of B is B[k*bk + j*bj], so
-// the same kernel serves X*W (forward), dY*W^T (input error) and X^T*dY
-// (weight gradient).  Operand lane layout of the 32x32x8 MFMA: lane l holds
-// A[l%32][4*(l/32)+0..3] and B[4*(l/32)+0..3][l%32]; accumulator v of lane l
-// is C[8*(v/4) + 4*(l/32) + v%4][l%32].
+// GEMM (gemm_tile<WT>): a 256-lane workgroup computes a (64 WT) x (64 WT)
+// tile of C; its four wavefronts each own a (32 WT) x (32 WT) quarter as
+// WT x WT accumulators of v_mfma_f32_32x32x16_bf16.  K advances 32 at a
+// time: the A and B tiles are staged in LDS k-contiguous ([row][k], rows
+// padded to 40 elements), filled from global memory with 16-byte loads
+// along whichever dimension of the operand is contiguous, the next slab's
+// loads in flight while the MFMAs consume the current one.  An operand is
+// either k-contiguous (A[i*ld + k], B[j*ld + k]) or not (A[k*ld + i],
+// B[k*ld + j]), so one kernel serves X*W (forward), dY*W^T (input error)
+// and X^T*dY (weight gradient).  MFMA lane layout: lane l holds
+// A[l%32][8*(l/32)+0..7] and B[8*(l/32)+0..7][l%32]; accumulator v of lane
+// l is C[8*(v/4) + 4*(l/32) + v%4][l%32].  Workgroups that share rows of A
+// are placed on one XCD (one L2).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -30,7 +36,7 @@
 
 namespace ctl {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -46,8 +52,11 @@ enum : int { EPI_STORE_F32 = 0, EPI_BIAS_RELU_BF16 = 1, EPI_BIAS_F32 = 2, EPI_MA
 struct GemmArgs {
     const uint16_t* A;
     const uint16_t* B;
-    long ai, ak, bk, bj;
+    long lda, ldb;
+    int a_kc, b_kc;            // operand contiguous along k
+    int a_vec, b_vec;          // 16-byte loads allowed (16-byte aligned base, ld a multiple of 8)
     int M, N, K;
+    int tiles_n;               // column tiles
     int epi;
     const float* bias;         // [N] (EPI_BIAS_*)
     const uint16_t* mask;      // [M][N] bf16 activation whose sign gates (EPI_MASK_BF16)
@@ -55,43 +64,150 @@ struct GemmArgs {
     uint16_t* Cb;              // [M][N] bf16 out
 };
 
-__global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int row0 = blockIdx.y * 64 + (wave >> 1) * 32;
-    const int col0 = blockIdx.x * 64 + (wave & 1) * 32;
-    const int r = row0 + (lane & 31);
-    const int c = col0 + (lane & 31);
-    const int kq = (lane >> 5) * 4;
-    f32x16 acc;
-    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
-    for (int k0 = 0; k0 < g.K; k0 += 8) {
-        bf16x4 a, b;
-        for (int q = 0; q < 4; ++q) {
-            const int k = k0 + kq + q;
-            a[q] = (r < g.M && k < g.K) ? (short)g.A[(long)r * g.ai + (long)k * g.ak] : (short)0;
-            b[q] = (c < g.N && k < g.K) ? (short)g.B[(long)k * g.bk + (long)c * g.bj] : (short)0;
-        }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+constexpr int GBK = 32;        // k per LDS slab
+constexpr int GKP = 40;        // padded LDS row (elements)
+
+// Eight consecutive elements of operand X along its contiguous dimension,
+// starting at (row r, k): (r, k..k+7) when kc, else (r..r+7, k); zero beyond
+// R rows / K columns.
+__device__ __forceinline__ uint4 gload8(const uint16_t* __restrict__ X, long ld, bool kc, bool vec, int r, int k,
+                                        int R, int K) {
+    uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    const uint16_t* p;
+    int lim;
+    if (kc) {
+        if (r >= R) return z;
+        p = X + (long)r * ld + k;
+        lim = K - k;
+    } else {
+        if (k >= K) return z;
+        p = X + (long)k * ld + r;
+        lim = R - r;
     }
-    for (int v = 0; v < 16; ++v) {
-        const int row = row0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (row >= g.M || c >= g.N) continue;
-        const long o = (long)row * g.N + c;
-        float x = acc[v];
-        switch (g.epi) {
-            case EPI_STORE_F32: g.Cf[o] = x; break;
-            case EPI_BIAS_RELU_BF16: x += g.bias[c]; g.Cb[o] = f2bf(x > 0.0f ? x : 0.0f); break;
-            case EPI_BIAS_F32: g.Cf[o] = x + g.bias[c]; break;
-            case EPI_MASK_BF16: g.Cb[o] = f2bf(bf2f(g.mask[o]) > 0.0f ? x : 0.0f); break;
+    if (vec && lim >= 8) return *reinterpret_cast<const uint4*>(p);
+    uint16_t e[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) e[q] = q < lim ? p[q] : (uint16_t)0;
+    z.x = e[0] | ((uint32_t)e[1] << 16);
+    z.y = e[2] | ((uint32_t)e[3] << 16);
+    z.z = e[4] | ((uint32_t)e[5] << 16);
+    z.w = e[6] | ((uint32_t)e[7] << 16);
+    return z;
+}
+
+// Rows [r0, r0 + BR) x k [k0, k0 + GBK) of an operand: BR*GBK/8 vectors,
+// BR/64 per lane.  Vector v covers (row v/4, k 8*(v%4)..) when kc, else
+// (rows 8*(v % (BR/8)).., k v / (BR/8)).
+template <int BR>
+__device__ __forceinline__ void gstage_load(uint4 (&reg)[BR / 64], const uint16_t* __restrict__ X, long ld, bool kc,
+                                            bool vec, int r0, int k0, int R, int K) {
+#pragma unroll
+    for (int q = 0; q < BR / 64; ++q) {
+        const int v = (int)threadIdx.x + 256 * q;
+        if (kc)
+            reg[q] = gload8(X, ld, true, vec, r0 + (v >> 2), k0 + (v & 3) * 8, R, K);
+        else
+            reg[q] = gload8(X, ld, false, vec, r0 + (v % (BR / 8)) * 8, k0 + v / (BR / 8), R, K);
+    }
+}
+
+template <int BR>
+__device__ __forceinline__ void gstage_store(const uint4 (&reg)[BR / 64], uint16_t* S, bool kc) {
+#pragma unroll
+    for (int q = 0; q < BR / 64; ++q) {
+        const int v = (int)threadIdx.x + 256 * q;
+        if (kc) {
+            *reinterpret_cast<uint4*>(S + (v >> 2) * GKP + (v & 3) * 8) = reg[q];
+        } else {
+            const int r = (v % (BR / 8)) * 8, k = v / (BR / 8);
+            const uint32_t w[4] = {reg[q].x, reg[q].y, reg[q].z, reg[q].w};
+#pragma unroll
+            for (int e = 0; e < 8; ++e) S[(r + e) * GKP + k] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
         }
     }
 }
 
+template <int WT>
+__global__ __launch_bounds__(256) void gemm_tile(GemmArgs g) {
+    constexpr int BM = 64 * WT;
+    __shared__ __align__(16) uint16_t As[BM * GKP];
+    __shared__ __align__(16) uint16_t Bs[BM * GKP];
+    // XCD-aware order: consecutive workgroup ids go to different XCDs; give
+    // each XCD a run of consecutive tiles (same rows of A, several columns).
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int full = nb / 8 * 8;
+    const int id = b < full ? (b % 8) * (full / 8) + b / 8 : b;
+    const int m0 = (id / g.tiles_n) * BM, n0 = (id % g.tiles_n) * BM;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32 * WT, wn = (wave & 1) * 32 * WT;
+    f32x16 acc[WT][WT];
+#pragma unroll
+    for (int a = 0; a < WT; ++a)
+#pragma unroll
+        for (int c = 0; c < WT; ++c)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[a][c][v] = 0.0f;
+    uint4 ra[BM / 64], rb[BM / 64];
+    gstage_load<BM>(ra, g.A, g.lda, g.a_kc, g.a_vec, m0, 0, g.M, g.K);
+    gstage_load<BM>(rb, g.B, g.ldb, g.b_kc, g.b_vec, n0, 0, g.N, g.K);
+    gstage_store<BM>(ra, As, g.a_kc);
+    gstage_store<BM>(rb, Bs, g.b_kc);
+    __syncthreads();
+    for (int k0 = 0; k0 < g.K; k0 += GBK) {
+        const bool more = k0 + GBK < g.K;
+        if (more) {
+            gstage_load<BM>(ra, g.A, g.lda, g.a_kc, g.a_vec, m0, k0 + GBK, g.M, g.K);
+            gstage_load<BM>(rb, g.B, g.ldb, g.b_kc, g.b_vec, n0, k0 + GBK, g.N, g.K);
+        }
+#pragma unroll
+        for (int ks = 0; ks < GBK; ks += 16) {
+            const int kk = ks + 8 * (lane >> 5);
+            bf16x8 fa[WT], fb[WT];
+#pragma unroll
+            for (int t = 0; t < WT; ++t) {
+                fa[t] = *reinterpret_cast<const bf16x8*>(As + (wm + 32 * t + (lane & 31)) * GKP + kk);
+                fb[t] = *reinterpret_cast<const bf16x8*>(Bs + (wn + 32 * t + (lane & 31)) * GKP + kk);
+            }
+#pragma unroll
+            for (int a = 0; a < WT; ++a)
+#pragma unroll
+                for (int c = 0; c < WT; ++c)
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[c], acc[a][c], 0, 0, 0);
+        }
+        __syncthreads();
+        if (more) {
+            gstage_store<BM>(ra, As, g.a_kc);
+            gstage_store<BM>(rb, Bs, g.b_kc);
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < WT; ++a)
+#pragma unroll
+        for (int c = 0; c < WT; ++c) {
+            const int col = n0 + wn + 32 * c + (lane & 31);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = m0 + wm + 32 * a + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                if (row >= g.M || col >= g.N) continue;
+                const long o = (long)row * g.N + col;
+                float x = acc[a][c][v];
+                switch (g.epi) {
+                    case EPI_STORE_F32: g.Cf[o] = x; break;
+                    case EPI_BIAS_RELU_BF16: x += g.bias[col]; g.Cb[o] = f2bf(x > 0.0f ? x : 0.0f); break;
+                    case EPI_BIAS_F32: g.Cf[o] = x + g.bias[col]; break;
+                    case EPI_MASK_BF16: g.Cb[o] = f2bf(bf2f(g.mask[o]) > 0.0f ? x : 0.0f); break;
+                }
+            }
+        }
+}
+
 // Loss gradient of the mean squared error over [B][N]: d = 2 (y - t) / (B N),
-// stored as BF16 for the backward GEMMs; per-block partial losses in FP32.
+// stored as BF16 for the backward GEMMs; per-block partial sums of squared
+// errors in FP32 (loss_part), or added to a device FP64 total (loss_acc).
 __global__ void mse_grad(const float* __restrict__ y, const float* __restrict__ t, uint16_t* __restrict__ d,
-                         float* __restrict__ dscale_out, float* __restrict__ loss_part, long total, float scale) {
+                         float* __restrict__ dscale_out, float* __restrict__ loss_part, double* __restrict__ loss_acc,
+                         long total, float scale) {
     __shared__ float red[256];
     float acc = 0.0f;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
@@ -106,7 +222,10 @@ __global__ void mse_grad(const float* __restrict__ y, const float* __restrict__ 
         if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) {
+        if (loss_part) loss_part[blockIdx.x] = red[0];
+        if (loss_acc) atomicAdd(loss_acc, (double)red[0]);
+    }
 }
 
 // db[j] = sum_b d[b][j] (d in BF16), FP32.
@@ -171,7 +290,8 @@ struct ctl_mlp {
     float* yout = nullptr;              // [cap][w_L]
     float* tgt = nullptr;
     float* part = nullptr;
-    int32_t* idx = nullptr;
+    int32_t* idx = nullptr;             // [ndata] epoch permutation
+    double* lacc = nullptr;             // epoch loss total
     // training data
     float *dX = nullptr, *dY = nullptr;
     long ndata = 0;
@@ -202,11 +322,31 @@ int malloc_dev(ctl_mlp* n, T** p, size_t count) {
     return 0;
 }
 
+// C[M][N] = A * B with element (i, k) of A at A[i*ai + k*ak] and (k, j) of
+// B at B[k*bk + j*bj]; one of each operand's strides is 1.
 void gemm(ctl_mlp* n, const uint16_t* A, long ai, long ak, const uint16_t* B, long bk, long bj, int M, int N, int K,
           int epi, const float* bias, const uint16_t* mask, float* Cf, uint16_t* Cb) {
-    GemmArgs g{A, B, ai, ak, bk, bj, M, N, K, epi, bias, mask, Cf, Cb};
-    dim3 grid((N + 63) / 64, (M + 63) / 64);
-    hipLaunchKernelGGL(gemm_bf16, grid, dim3(256), 0, n->stream, g);
+    GemmArgs g{};
+    g.A = A;
+    g.B = B;
+    g.a_kc = ak == 1;
+    g.lda = g.a_kc ? ai : ak;
+    g.b_kc = bk == 1;
+    g.ldb = g.b_kc ? bj : bk;
+    g.a_vec = (g.lda % 8 == 0) && ((uintptr_t)A % 16 == 0);
+    g.b_vec = (g.ldb % 8 == 0) && ((uintptr_t)B % 16 == 0);
+    g.M = M; g.N = N; g.K = K;
+    g.epi = epi; g.bias = bias; g.mask = mask; g.Cf = Cf; g.Cb = Cb;
+    // 128 x 128 tiles when they still give every CU a workgroup, else 64 x 64
+    const long big = (long)((M + 127) / 128) * ((N + 127) / 128);
+    if (big >= 256) {
+        g.tiles_n = (N + 127) / 128;
+        hipLaunchKernelGGL(gemm_tile<2>, dim3((unsigned)big), dim3(256), 0, n->stream, g);
+    } else {
+        g.tiles_n = (N + 63) / 64;
+        const long tiles = (long)((M + 63) / 64) * g.tiles_n;
+        hipLaunchKernelGGL(gemm_tile<1>, dim3((unsigned)tiles), dim3(256), 0, n->stream, g);
+    }
 }
 
 int ensure_cap(ctl_mlp* n, int B) {
@@ -220,7 +360,7 @@ int ensure_cap(ctl_mlp* n, int B) {
     if ((rc = malloc_dev(n, &n->err0, (size_t)B * wmax)) || (rc = malloc_dev(n, &n->err1, (size_t)B * wmax)) ||
         (rc = malloc_dev(n, &n->yout, (size_t)B * n->widths[n->L])) ||
         (rc = malloc_dev(n, &n->tgt, (size_t)B * n->widths[n->L])) || (rc = malloc_dev(n, &n->part, 1024)) ||
-        (rc = malloc_dev(n, &n->idx, B)))
+        (rc = malloc_dev(n, &n->lacc, 1)))
         return rc;
     n->cap = B;
     return 0;
@@ -261,6 +401,7 @@ float loss_and_grad(ctl_mlp* n, int B, const float* t_dev, float* dscale_dbg) {
     const long total = (long)B * n->widths[n->L];
     const int blocks = (int)std::min<long>(1024, (total + 255) / 256);
     hipLaunchKernelGGL(mse_grad, dim3(blocks), dim3(256), 0, n->stream, n->yout, t_dev, n->err0, dscale_dbg, n->part,
+                       (double*)nullptr,
                        total, 1.0f / (float)total);
     std::vector<float> part(blocks);
     if (hipMemcpyAsync(part.data(), n->part, sizeof(float) * blocks, hipMemcpyDeviceToHost, n->stream) != hipSuccess)
@@ -442,6 +583,8 @@ int ctl_mlp_set_data(ctl_mlp* n, const float* x, const float* t, int64_t rows) {
     MHIP(hipMalloc(&n->dY, sizeof(float) * rows * n->widths[n->L]), "hipMalloc");
     MHIP(hipMemcpy(n->dX, x, sizeof(float) * rows * n->widths[0], hipMemcpyHostToDevice), "H2D");
     MHIP(hipMemcpy(n->dY, t, sizeof(float) * rows * n->widths[n->L], hipMemcpyHostToDevice), "H2D");
+    if (n->idx) { (void)hipFree(n->idx); n->idx = nullptr; }
+    MHIP(hipMalloc(&n->idx, sizeof(int32_t) * rows), "hipMalloc");
     n->ndata = rows;
     return 0;
 }
@@ -459,29 +602,35 @@ int ctl_mlp_train(ctl_mlp* n, int32_t epochs, int32_t batch, float lr, uint64_t 
     std::vector<int32_t> perm(n->ndata);
     for (long i = 0; i < n->ndata; ++i) perm[i] = (int32_t)i;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const int w0 = n->widths[0], wL = n->widths[n->L];
+    const long total = (long)batch * wL;
+    const int blocks = (int)std::min<long>(1024, (total + 255) / 256);
     for (int ep = 0; ep < epochs; ++ep) {
+        // one upload of the epoch's permutation; the loss stays on the device
+        // until the epoch ends, so the batches run without host round trips
         std::shuffle(perm.begin(), perm.end(), rng);
-        double acc = 0.0;
+        MHIP(hipMemcpyAsync(n->idx, perm.data(), sizeof(int32_t) * n->ndata, hipMemcpyHostToDevice, n->stream), "H2D");
+        MHIP(hipMemsetAsync(n->lacc, 0, sizeof(double), n->stream), "memset");
         long nb = 0;
         for (long s0 = 0; s0 + batch <= n->ndata; s0 += batch) {
-            MHIP(hipMemcpyAsync(n->idx, perm.data() + s0, sizeof(int32_t) * batch, hipMemcpyHostToDevice, n->stream), "H2D");
-            const int w0 = n->widths[0], wL = n->widths[n->L];
             hipLaunchKernelGGL(gather_rows, dim3((unsigned)(((long)batch * w0 + 255) / 256)), dim3(256), 0, n->stream,
-                               n->dX, n->idx, n->act[0], (float*)nullptr, batch, w0);
+                               n->dX, n->idx + s0, n->act[0], (float*)nullptr, batch, w0);
             hipLaunchKernelGGL(gather_rows, dim3((unsigned)(((long)batch * wL + 255) / 256)), dim3(256), 0, n->stream,
-                               n->dY, n->idx, (uint16_t*)nullptr, n->tgt, batch, wL);
+                               n->dY, n->idx + s0, (uint16_t*)nullptr, n->tgt, batch, wL);
             forward(n, batch);
-            const float l = loss_and_grad(n, batch, n->tgt, nullptr);
+            hipLaunchKernelGGL(mse_grad, dim3(blocks), dim3(256), 0, n->stream, n->yout, n->tgt, n->err0,
+                               (float*)nullptr, (float*)nullptr, n->lacc, total, 1.0f / (float)total);
             backward(n, batch);
             n->step += 1;
             const float c1 = 1.0f - std::pow(b1, (float)n->step), c2 = 1.0f - std::pow(b2, (float)n->step);
             hipLaunchKernelGGL(adam, dim3((unsigned)((n->nparam + 255) / 256)), dim3(256), 0, n->stream, n->w, n->wb,
                                n->g, n->m, n->v, n->nparam, lr, b1, b2, eps, c1, c2);
-            acc += l;
             ++nb;
         }
+        double sum = 0.0;
+        MHIP(hipMemcpyAsync(&sum, n->lacc, sizeof(double), hipMemcpyDeviceToHost, n->stream), "D2H");
         MHIP(hipStreamSynchronize(n->stream), "train");
-        if (losses) losses[ep] = nb ? (float)(acc / nb) : NAN;
+        if (losses) losses[ep] = nb ? (float)(sum / ((double)total * nb)) : NAN;
     }
     return 0;
 }
@@ -492,6 +641,7 @@ void ctl_mlp_destroy(ctl_mlp* n) {
     for (void* q : n->allocs) (void)hipFree(q);
     if (n->dX) (void)hipFree(n->dX);
     if (n->dY) (void)hipFree(n->dY);
+    if (n->idx) (void)hipFree(n->idx);
     if (n->stream) (void)hipStreamDestroy(n->stream);
     delete n;
 }

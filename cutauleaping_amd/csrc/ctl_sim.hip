@@ -140,14 +140,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
     store_lane(p, L);
 }
 
-const void* group_kernel_fn(int G) {
+const void* group_kernel_fn(int G, bool rg) {
     switch (G) {
-        case 2: return (const void*)group_kernel<2>;
-        case 4: return (const void*)group_kernel<4>;
-        case 8: return (const void*)group_kernel<8>;
-        case 16: return (const void*)group_kernel<16>;
-        case 32: return (const void*)group_kernel<32>;
-        case 64: return (const void*)group_kernel<64>;
+        case 2: return rg ? (const void*)group_kernel<2, true> : (const void*)group_kernel<2, false>;
+        case 4: return rg ? (const void*)group_kernel<4, true> : (const void*)group_kernel<4, false>;
+        case 8: return rg ? (const void*)group_kernel<8, true> : (const void*)group_kernel<8, false>;
+        case 16: return rg ? (const void*)group_kernel<16, true> : (const void*)group_kernel<16, false>;
+        case 32: return rg ? (const void*)group_kernel<32, true> : (const void*)group_kernel<32, false>;
+        case 64: return (const void*)group_kernel<64, false>;
     }
     return nullptr;
 }
@@ -283,7 +283,7 @@ struct ctl_sim {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int n = 0, ns = 0, block = 256, grid = 0;
     int lanes = 1;                   // lanes per simulation (1: tau_leap_kernel, else group_kernel<lanes>)
-    int compact = 0;                 // one-lane kernel: iterations between regroupings by mode (0: off)
+    int compact = 0;                 // iterations between regroupings by mode (0: off)
     bool lds = false;
     size_t smem = 0;
     KParams kp{};
@@ -416,6 +416,24 @@ int ctl_device_count(void) {
     return n;
 }
 
+int ctl_phase_clock(int32_t gpu, uint64_t* out, int32_t count, int32_t reset) {
+    ctl::clear_error();
+#if CTL_PHASE_CLOCK
+    if (!out || count < ctl::PH_COUNT) return ctl::fail(CTL_E_ARGS, "need a buffer of CTL_PHASES entries");
+    HIPCHK(hipSetDevice(gpu), "hipSetDevice");
+    HIPCHK(hipMemcpyFromSymbol(out, HIP_SYMBOL(ctl::phase_total), sizeof(uint64_t) * ctl::PH_COUNT),
+           "hipMemcpyFromSymbol");
+    if (reset) {
+        const uint64_t zero[ctl::PH_COUNT] = {};
+        HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(ctl::phase_total), zero, sizeof zero), "hipMemcpyToSymbol");
+    }
+    return 0;
+#else
+    (void)gpu; (void)out; (void)count; (void)reset;
+    return ctl::fail(CTL_E_STATE, "built without the phase clock: load libctl_phases.so");
+#endif
+}
+
 int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     ctl::clear_error();
     if (!mh || !o || !out) return ctl::fail(CTL_E_ARGS, "null argument");
@@ -492,7 +510,10 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
         s->block = gpb * G;
         s->smem = (size_t)gpb * gwords * 4;
         s->lds = true;
-        const void* kfn = group_kernel_fn(G);
+        // regrouping simulations by mode (leap / SSA) every `compact` iterations
+        const int compact = o->compact_interval > 0 ? o->compact_interval : (o->compact_interval < 0 ? 0 : 8);
+        s->compact = compact > 0 && G < 64 && s->block > 64 ? compact : 0;
+        const void* kfn = group_kernel_fn(G, s->compact > 0);
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, s->block, s->smem);
         if (e != hipSuccess || per_cu <= 0) per_cu = 1;
         long long groups = o->slots > 0 ? (long long)o->slots : (long long)prop.multiProcessorCount * per_cu * gpb;
@@ -643,7 +664,7 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     p.ssa_steps = o->ssa_steps;
     p.ssa_factor = o->ssa_factor;
     p.force_ssa = o->force_ssa ? 1 : 0;
-    p.ssa_chunk = o->ssa_chunk > 0 ? o->ssa_chunk : 10;
+    p.ssa_chunk = o->ssa_chunk > 0 ? o->ssa_chunk : (G > 1 ? 6 : 10);
     p.compact = s->compact;
     p.iters = o->iters_per_launch > 0 ? o->iters_per_launch : (G > 1 ? 65536 : 4096);
     p.ns = ns;
@@ -706,13 +727,13 @@ int ctl_sim_run(ctl_sim* s) {
                 else
                     hipLaunchKernelGGL(tau_leap_kernel<false>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p);
                 break;
-            case 2: hipLaunchKernelGGL(group_kernel<2>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
-            case 4: hipLaunchKernelGGL(group_kernel<4>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
-            case 8: hipLaunchKernelGGL(group_kernel<8>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
-            case 16: hipLaunchKernelGGL(group_kernel<16>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
-            case 32: hipLaunchKernelGGL(group_kernel<32>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
-            case 64: hipLaunchKernelGGL(group_kernel<64>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p); break;
-            default: return ctl::fail(CTL_E_STATE, "bad lanes_per_sim");
+            default: {
+                const void* fn = group_kernel_fn(s->lanes, s->compact > 0);
+                if (!fn) return ctl::fail(CTL_E_STATE, "bad lanes_per_sim");
+                void* args[] = {&p};
+                HIPCHK(hipLaunchKernel(fn, dim3(s->grid), dim3(s->block), args, s->smem, s->stream),
+                       "group_kernel launch");
+            }
         }
         HIPCHK(hipGetLastError(), "tau_leap_kernel launch");
         HIPCHK(hipEventRecord(s->ev1, s->stream), "hipEventRecord");

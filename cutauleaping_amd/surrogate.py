@@ -183,18 +183,22 @@ def screened_sweep(problem: SweepProblem, surrogate: Surrogate, u, *, replicas: 
 def sweep_study(problem: SweepProblem, *, n_train=8192, n_test=512, replicas=256, judge_replicas=None,
                 members=5, epochs=150, batch=256, lr=1e-3, quantiles=(0.9, 0.95, 0.98, 0.99, 1.0), seed=1,
                 gpu=0, log=None, sim_opts=None, n_val=1024, target=0.05, policy=0.95) -> dict:
-    """Data generation, ensemble training and a judged, timed sweep.
+    """Data generation, ensemble training, threshold selection on validation
+    sets and a judged, timed sweep of held-out test sets.
 
     Training: n_train parameter sets x `replicas` simulations, reduced to
-    mean/sd on the device.  Held out: n_test new sets, simulated with
-    `judge_replicas` (default `replicas`) to judge, and with `replicas`
-    (timed) as the plain sweep.  Fallback thresholds are quantiles of the
-    ensemble spread over n_val further sets (prediction only; quantile 1.0
-    = no fallback); the screened sweep is timed and judged at each.  Errors
-    are relative L2 per set over all times and species, of the means and of
-    the standard deviations, averaged over the held-out sets.  The chosen
-    operating point is the `policy` quantile (fallback active) when its mean
-    errors are both <= target, else the fastest point that meets the target."""
+    mean/sd on the device.  Validation: n_val further sets, judged with
+    `judge_replicas` simulations each and answered plainly with `replicas`
+    (the fallback's answer).  For each candidate threshold (a quantile of the
+    ensemble spread over the validation sets; quantile 1.0 = no fallback) the
+    screened answer's errors are measured on the validation sets; the chosen
+    threshold is the `policy` quantile when both its mean errors are <=
+    target, else the one with the fewest fallbacks that meets the target,
+    else the most accurate.  Test: n_test new sets, judged likewise, answered
+    by the plain sweep (timed) and by the screened sweep at the chosen
+    threshold only (timed); the reported speed-up and errors are those of the
+    test sets.  Errors are relative L2 per set over all times and species, of
+    the means and of the standard deviations, averaged over the sets."""
     sim_opts = dict(sim_opts or {})
     judge_replicas = judge_replicas or replicas
     say = log or (lambda *a: None)
@@ -202,65 +206,78 @@ def sweep_study(problem: SweepProblem, *, n_train=8192, n_test=512, replicas=256
     u_tr = problem.sample_u(n_train, rng)
     u_te = problem.sample_u(n_test, rng)
     u_val = problem.sample_u(n_val, rng)
+
+    def simulate_all(u, reps, sd_seed, what):
+        chunk = max(1, (1 << 19) // reps)           # sets per launch: about 2^19 simulations
+        means, sds, fir = [], [], 0
+        for i in range(0, len(u), chunk):
+            m, s, st, _ = problem.simulate(u[i:i + chunk], reps, offset=i * reps, seed=sd_seed, gpu=gpu,
+                                           **sim_opts)
+            means.append(m)
+            sds.append(s)
+            fir += st["firings"]
+            say(f"# {what} {min(i + chunk, len(u))}/{len(u)} sets")
+        return np.concatenate(means), np.concatenate(sds), fir
+
     t0 = time.perf_counter()
-    chunk = max(1, (1 << 19) // replicas)           # sets per launch: about 2^19 simulations
-    means, sds, gen_fir = [], [], 0
-    for i in range(0, n_train, chunk):
-        m, s, st, _ = problem.simulate(u_tr[i:i + chunk], replicas, offset=i * replicas, seed=seed, gpu=gpu,
-                                       **sim_opts)
-        means.append(m)
-        sds.append(s)
-        gen_fir += st["firings"]
-        say(f"# training data {min(i + chunk, n_train)}/{n_train} sets")
+    m_tr, s_tr, gen_fir = simulate_all(u_tr, replicas, seed, "training data")
     gen_s = time.perf_counter() - t0
-    y_tr = problem.features(np.concatenate(means), np.concatenate(sds))
+    y_tr = problem.features(m_tr, s_tr)
     sur = Surrogate(WIDTHS if problem.n_in == 32 and problem.n_out == 1000 else
                     (problem.n_in, 256, 256, 256, problem.n_out), members=members, gpu=gpu, seed=seed)
     t0 = time.perf_counter()
     losses = sur.fit(u_tr, y_tr, epochs=epochs, batch=batch, lr=lr, seed=seed)
     train_s = time.perf_counter() - t0
     say(f"# trained {members} members in {train_s:.1f}s, final losses {losses[:, -1].round(4).tolist()}")
-    # judge values, then the plain sweep (timed) on the held-out sets
-    jm, js = [], []
-    jchunk = max(1, (1 << 19) // judge_replicas)
-    for i in range(0, n_test, jchunk):
-        m, s, _, _ = problem.simulate(u_te[i:i + jchunk], judge_replicas, offset=(1 << 30) + i * judge_replicas,
-                                      seed=seed + 1, gpu=gpu, **sim_opts)
-        jm.append(m)
-        js.append(s)
-        say(f"# judge {min(i + jchunk, n_test)}/{n_test} sets")
-    jm, js = np.concatenate(jm), np.concatenate(js)
-    say(f"# judged {n_test} sets with {judge_replicas} replicas")
-    pm, ps, _, plain_s = problem.simulate(u_te, replicas, offset=(1 << 29), seed=seed + 2, gpu=gpu, **sim_opts)
-    say(f"# plain sweep of {n_test} sets x {replicas} replicas: {plain_s:.2f}s")
-    _, spread_val = sur.predict(u_val)
+
+    # threshold selection on the validation sets
+    vm, vs, _ = simulate_all(u_val, judge_replicas, seed + 4, "validation judge")
+    pvm, pvs, _ = simulate_all(u_val, replicas, seed + 5, "validation plain")
+    y_val, spread_val = sur.predict(u_val)
+    svm, svs = problem.unfeatures(y_val)
     points = []
     for q in quantiles:
         thr = float(np.quantile(spread_val, q)) if q < 1.0 else float("inf")
-        res = screened_sweep(problem, sur, u_te, replicas=replicas, threshold=thr, offset=(1 << 28),
-                             seed=seed + 3, gpu=gpu, **sim_opts)
-        e_m = relative_error(res["mean"], jm)
-        e_s = relative_error(res["sd"], js)
-        points.append(dict(quantile=q, threshold=thr, seconds=res["seconds"], surrogate_s=res["surrogate_seconds"],
-                           fallback_s=res["simulator_seconds"], fallback_fraction=float(res["fallback"].mean()),
-                           speedup=plain_s / res["seconds"], err_mean=float(e_m.mean()), err_sd=float(e_s.mean()),
+        fb = spread_val > thr
+        e_m = relative_error(np.where(fb[:, None, None], pvm, svm), vm)
+        e_s = relative_error(np.where(fb[:, None, None], pvs, svs), vs)
+        points.append(dict(quantile=q, threshold=thr, fallback_fraction=float(fb.mean()),
+                           err_mean=float(e_m.mean()), err_sd=float(e_s.mean()),
                            err_mean_p95=float(np.percentile(e_m, 95)), err_sd_p95=float(np.percentile(e_s, 95))))
-        say(f"# quantile {q}: speed-up {points[-1]['speedup']:.1f}x, fallback {points[-1]['fallback_fraction']:.3f}, "
+        say(f"# validation quantile {q}: fallback {points[-1]['fallback_fraction']:.3f}, "
             f"errors {points[-1]['err_mean']:.4f} / {points[-1]['err_sd']:.4f}")
     ok = [p for p in points if p["err_mean"] <= target and p["err_sd"] <= target]
     pol = [p for p in ok if p["quantile"] == policy]
-    best = pol[0] if pol else (max(ok, key=lambda p: p["speedup"]) if ok else min(points, key=lambda p: p["err_sd"]))
+    chosen = pol[0] if pol else (min(ok, key=lambda p: p["fallback_fraction"]) if ok else
+                                 min(points, key=lambda p: p["err_sd"]))
+
+    # the test sets: judge, plain sweep (timed), screened sweep at the chosen threshold (timed)
+    jm, js, _ = simulate_all(u_te, judge_replicas, seed + 1, "test judge")
+    pm, ps, _, plain_s = problem.simulate(u_te, replicas, offset=0, seed=seed + 2, gpu=gpu, **sim_opts)
+    say(f"# plain sweep of {n_test} sets x {replicas} replicas: {plain_s:.2f}s")
+    res = screened_sweep(problem, sur, u_te, replicas=replicas, threshold=chosen["threshold"], offset=0,
+                         seed=seed + 3, gpu=gpu, **sim_opts)
+    e_m = relative_error(res["mean"], jm)
+    e_s = relative_error(res["sd"], js)
+    test = dict(quantile=chosen["quantile"], threshold=chosen["threshold"], seconds=res["seconds"],
+                surrogate_s=res["surrogate_seconds"], fallback_s=res["simulator_seconds"],
+                fallback_fraction=float(res["fallback"].mean()), speedup=plain_s / res["seconds"],
+                err_mean=float(e_m.mean()), err_sd=float(e_s.mean()),
+                err_mean_p95=float(np.percentile(e_m, 95)), err_sd_p95=float(np.percentile(e_s, 95)))
+    say(f"# test at validation quantile {chosen['quantile']}: speed-up {test['speedup']:.1f}x, fallback "
+        f"{test['fallback_fraction']:.3f}, errors {test['err_mean']:.4f} / {test['err_sd']:.4f}")
     # the trivial predictor (the training mean) for scale, and the plain sweep's own error
     mm, ms = problem.unfeatures(np.repeat(y_tr.mean(axis=0, keepdims=True), n_test, axis=0))
     sur.close()
     return dict(
-        n_train=n_train, n_test=n_test, replicas=replicas, judge_replicas=judge_replicas, members=members,
-        generation_s=gen_s, generation_firings=int(gen_fir), train_s=train_s,
-        final_loss=float(losses[:, -1].mean()), plain_sweep_s=plain_s, points=points, chosen=best,
-        speedup=best["speedup"], screened_sweep_s=best["seconds"], fallback_fraction=best["fallback_fraction"],
-        err_mean=best["err_mean"], err_sd=best["err_sd"], meets_target=bool(ok),
-        surrogate_only_err_mean=points[-1]["err_mean"] if quantiles[-1] == 1.0 else None,
-        surrogate_only_err_sd=points[-1]["err_sd"] if quantiles[-1] == 1.0 else None,
+        n_train=n_train, n_test=n_test, n_val=n_val, replicas=replicas, judge_replicas=judge_replicas,
+        members=members, generation_s=gen_s, generation_firings=int(gen_fir), train_s=train_s,
+        final_loss=float(losses[:, -1].mean()), plain_sweep_s=plain_s, validation=points, chosen=chosen,
+        test=test, speedup=test["speedup"], screened_sweep_s=test["seconds"],
+        fallback_fraction=test["fallback_fraction"], err_mean=test["err_mean"], err_sd=test["err_sd"],
+        meets_target=bool(test["err_mean"] <= target and test["err_sd"] <= target),
+        surrogate_only_val_err_mean=points[-1]["err_mean"] if quantiles[-1] == 1.0 else None,
+        surrogate_only_val_err_sd=points[-1]["err_sd"] if quantiles[-1] == 1.0 else None,
         trivial_err_mean=float(relative_error(mm, jm).mean()),
         trivial_err_sd=float(relative_error(ms, js).mean()),
         plain_err_mean=float(relative_error(pm, jm).mean()),

@@ -79,14 +79,17 @@ typedef struct {
     int32_t  block_size;       /* threads per workgroup */
     int32_t  slots;            /* resident simulation slots */
     int32_t  iters_per_launch; /* leap/SSA iterations per slot per launch */
-    int32_t  ssa_chunk;        /* SSA steps per iteration of a slot */
+    int32_t  ssa_chunk;        /* SSA steps per iteration of a slot (0: 10 for one lane per
+                                  simulation, 6 for a regrouping group kernel) */
     int32_t  lanes_per_sim;    /* 1: one lane runs one simulation; 2..64 (power of two): that many
                                   lanes of a wavefront cooperate on one simulation whose state
                                   stays in LDS (the group algorithm); 0: chosen by the library */
-    int32_t  compact_interval; /* one lane per simulation: every this many iterations a workgroup
-                                  of several wavefronts regroups its simulations so that leap-mode
-                                  and SSA-mode ones run in different wavefronts (<= 0: never, the
-                                  default); trajectories do not depend on it */
+    int32_t  compact_interval; /* every this many iterations a workgroup of several wavefronts
+                                  regroups its simulations so that leap-mode and SSA-mode ones run
+                                  in different wavefronts, SSA bursts then running ssa_chunk steps
+                                  per iteration; trajectories do not depend on it.  0: library
+                                  default (8 for 2..32 lanes per simulation, never for one lane),
+                                  < 0: never */
 } ctl_opts;
 
 typedef struct {
@@ -247,6 +250,13 @@ void ctl_sim_destroy(ctl_sim* s);
 typedef struct ctl_batch ctl_batch;
 int ctl_batch_create(const ctl_model* const* models, const int64_t* replicas, int32_t nmodels,
                      const ctl_opts* o, ctl_batch** out);
+/* ctl_batch_create with lanes per simulation chosen per model: lanes[i] > 0
+ * runs model i with that many lanes per simulation (its kernel class),
+ * lanes[i] == 0 leaves it to o->lanes_per_sim / the library's size rule.
+ * No counterpart in the reference (no batching there); autotune.tune_batch
+ * produces the array. */
+int ctl_batch_create_lanes(const ctl_model* const* models, const int64_t* replicas, const int32_t* lanes,
+                           int32_t nmodels, const ctl_opts* o, ctl_batch** out);
 int ctl_batch_run(ctl_batch* b);
 /* Samples of one model, [replica][sample][sampled species]. */
 int ctl_batch_samples(const ctl_batch* b, int32_t model, uint32_t* out, size_t count);
@@ -297,6 +307,16 @@ const char* ctl_last_error(void);
 const char* ctl_version(void);
 /* Number of HIP devices visible (0 when none); never fails loudly. */
 int ctl_device_count(void);
+/* Profiling build only (libctl_phases.so, compiled with CTL_PHASE_CLOCK=1):
+ * out[0 .. CTL_PHASES) receives the shader-clock ticks the group kernel's
+ * wavefronts spent in each phase since the last reset, in the order
+ * propensities, tau', switch / tau'' / pick, counts and gaps, Poisson queue,
+ * update, sampling, SSA burst opening, SSA select, SSA update, SSA
+ * dependents, before a leap, before SSA steps, loop (fetch / finish),
+ * regrouping, parking.  reset != 0 zeroes the totals after reading.  The
+ * normal library returns CTL_E_STATE.  No counterpart in the reference. */
+#define CTL_PHASES 16
+int ctl_phase_clock(int32_t gpu, uint64_t* out, int32_t count, int32_t reset);
 
 #ifdef __cplusplus
 }

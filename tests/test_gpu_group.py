@@ -199,18 +199,87 @@ def test_group_batch_uneven_models_bit_exact(gpu, oracle_lib, force):
 
 
 def test_autotune_picks_a_measured_candidate(gpu, tmp_path):
-    """The autotuner times every lanes-per-simulation candidate on a short
-    probe of the model and returns the fastest; the result is memoised on disk."""
+    """The autotuner times every lanes-per-simulation candidate on probe
+    windows spread over the horizon, refines the winner's regrouping
+    interval, SSA chunk and iteration budget, and returns the fastest
+    configuration measured; the result is memoised on disk."""
     from cutauleaping_amd import autotune, networks
     print("case: autotune", flush=True)
     net = networks.mass_action(100, 200, seed=1)
     ts = networks.sample_times(200, 50.0)
     cache = str(tmp_path / "tune.json")
-    res = autotune.tune(net["left"], net["right"], net["x0"], net["c"], ts, probe_sims=8192, probe_fraction=0.02,
-                        candidates=(1, 8, 16, 64), cache_file=cache)
-    assert res["lanes_per_sim"] in (1, 8, 16, 64)
-    assert set(res["scores"]) == {1, 8, 16, 64} and min(res["scores"].values()) > 0
-    assert res["scores"][res["lanes_per_sim"]] == max(res["scores"].values())
+    res = autotune.tune(net["left"], net["right"], net["x0"], net["c"], ts, probe_sims=8192, probe_fraction=0.01,
+                        windows=3, candidates=(1, 8, 16, 64), cache_file=cache, verbose=True)
+    assert res["lanes_per_sim"] in (1, 8, 16, 64) and res["options"]["lanes_per_sim"] == res["lanes_per_sim"]
+    stage1 = [res["scores"][f"lanes_per_sim={g}"] for g in (1, 8, 16, 64)]
+    assert min(stage1) > 0
+    assert len(res["scores"]) > 4                          # stage 2 ran
+    assert res["scores"][autotune._key(res["options"])] == max(res["scores"].values())
+    assert res["probe"]["windows"] == 3 and res["probe"]["window_starts"][-1] > 30.0
     autotune._memo.clear()
-    again = autotune.tune(net["left"], net["right"], net["x0"], net["c"], ts, cache_file=cache)
-    assert again["lanes_per_sim"] == res["lanes_per_sim"]
+    again = autotune.tune(net["left"], net["right"], net["x0"], net["c"], ts, probe_sims=8192, probe_fraction=0.01,
+                          windows=3, candidates=(1, 8, 16, 64), cache_file=cache)
+    assert again["options"] == res["options"]
+
+
+def test_autotune_batch_classes(gpu, oracle_lib):
+    """tune_batch probes each kernel class's largest model at its G and the
+    neighbouring G, and returns lanes per model that the batch accepts; the
+    batch then runs each model with those lanes, bit-exact against the
+    oracle's algorithm for that G."""
+    from cutauleaping_amd import autotune, networks
+    print("case: autotune batch classes", flush=True)
+    ts = networks.sample_times(20, 0.05)
+    nets = []
+    for i in (3, 17, 40, 51, 77, 90):
+        n = networks.batched_model(i, seed=2, t_end=0.05)
+        n["ts"] = ts
+        n["replicas"] = min(n["replicas"], 256)
+        nets.append(n)
+    reps = [n["replicas"] for n in nets]
+    res = autotune.tune_batch(nets, reps, probe_fraction=0.5, verbose=True)
+    lanes = res["lanes"]
+    assert lanes.shape == (len(nets),) and set(int(g) for g in lanes) <= {1, 2, 4, 8, 16, 32, 64}
+    for G, cl in res["classes"].items():
+        assert cl["chosen"] in cl["scores"] and cl["scores"][cl["chosen"]] == max(cl["scores"].values())
+    models = [gpu.Model.from_arrays(n["left"], n["right"], n["x0"], n["c"], ts, cs=n["cs"]) for n in nets]
+    with gpu.Batch(models, reps, seed=0xB00, lanes=lanes) as b:
+        b.run()
+        for i, n in enumerate(nets):
+            assert b.model_lanes(i) == lanes[i]
+            G = int(lanes[i])
+            k = min(reps[i], 16)          # the first k replicas (each has its own stream)
+            out, st = oracle_lib.run(n["left"], n["right"], n["x0"], n["c"], ts, cs=n["cs"], n=k, seed=0xB00,
+                                     sim_base=b.stream_base(i), group=G if G > 1 else 0, threads=8)
+            assert np.array_equal(b.samples(i)[:k], out), (i, G)
+    for m in models:
+        m.close()
+
+
+@pytest.mark.parametrize("seed,G,compact,chunk,iters,slots,force", [
+    (40, 2, 1, 1, 5, 0, False),
+    (42, 4, 3, 7, 11, 0, True),
+    (43, 8, 8, 6, 0, 0, False),
+    (49, 16, 2, 3, 9, 24, False),
+    (47, 32, 5, 100, 13, 0, False),
+    (40, 16, -1, 0, 0, 0, False),
+])
+def test_group_regrouping_bit_exact(gpu, oracle_lib, seed, G, compact, chunk, iters, slots, force):
+    """A workgroup's simulations are reassigned to its wavefronts by mode
+    (leap / SSA) every `compact` iterations, SSA bursts running `chunk` steps
+    per iteration; with small iteration budgets and slot counts that park
+    simulations (and half-run bursts) across launches, every sample and
+    counter equals the oracle's, which knows no scheduling."""
+    print(f"case: group regrouping seed={seed} G={G} compact={compact} chunk={chunk} iters={iters} "
+          f"slots={slots} force={force}", flush=True)
+    net, feed, x0, c, cs, ts = _random_case(seed, 14, 30, 200, True)
+    m = gpu.Model.from_arrays(net["left"], net["right"], x0, c, ts, feed=feed, cs=cs)
+    with gpu.Sim(m, 200, 1, seed=seed, lanes_per_sim=G, force_ssa=force, compact_interval=compact,
+                 ssa_chunk=chunk, iters_per_launch=iters, slots=slots) as s:
+        s.run()
+        out, st, xf = oracle_lib.run(net["left"], net["right"], x0, c, ts, locked=feed, cs=cs, n=200,
+                                     seed=seed, force_ssa=force, final=True, group=G)
+        if not force:       # both modes present, so regrouping has something to separate
+            assert st["leaps"].sum() > 0 and st["ssa_steps"].sum() > 0
+        _compare(s, out, st, xf)
+    m.close()
