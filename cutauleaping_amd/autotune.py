@@ -15,9 +15,10 @@ actual device and keeps the fastest configuration:
 2. Stage 1 times every lanes-per-simulation candidate (1, and G = 2..64 where
    the model fits the group kernel's LDS) with the library's defaults.
 3. Stage 2 refines the winner: for G = 2..32 the regrouping interval
-   (compact_interval, including off) and the SSA chunk; for G = 1 the
-   workgroup size with and without regrouping, then the SSA chunk; for
-   every G the iteration budget per launch.
+   (compact_interval, including off) and the SSA chunk, then a workgroup of
+   two wavefronts instead of four; for G = 1 the workgroup size with and
+   without regrouping, then the SSA chunk; for every G the iteration budget
+   per launch.
 
 A configuration's score is its total firings over its total kernel time
 across the windows (firings per second of the whole probe).  Results are
@@ -163,6 +164,11 @@ def tune(left, right, x0, c, ts, *, feed=None, cs=None, gpu=0, probe_sims=65536,
                 grid = []
             for extra in grid:
                 o = dict(best, **extra)
+                r = score(o)
+                if r > best_rate:
+                    best, best_rate = o, r
+            if 1 < G < 64 and best.get("compact_interval", 0) >= 0:
+                o = dict(best, block_size=128)      # regrouping over two wavefronts instead of four
                 r = score(o)
                 if r > best_rate:
                     best, best_rate = o, r

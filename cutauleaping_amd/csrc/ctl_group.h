@@ -77,15 +77,23 @@ static __shared__ unsigned long long phase_row[4][PH_COUNT + 1];    // per wavef
 __device__ __forceinline__ bool phase_leader() {
     return (int)(threadIdx.x & 63u) == __builtin_ffsll((long long)__ballot(1)) - 1;
 }
+// The clock read is a volatile asm with a memory clobber, so that the
+// compiler neither merges the reads of two branches nor moves the row
+// updates across them (a predicated if/else would interleave both sides).
+__device__ __forceinline__ unsigned long long phase_now() {
+    unsigned long long t;
+    asm volatile("s_memtime %0\n\ts_waitcnt lgkmcnt(0)" : "=s"(t) : : "memory");
+    return t;
+}
 __device__ __forceinline__ void phase_start() {
     if (phase_leader()) {
         unsigned long long* r = phase_row[threadIdx.x >> 6];
         for (int k = 0; k < PH_COUNT; ++k) r[k] = 0ull;
-        r[PH_COUNT] = __builtin_amdgcn_s_memtime();
+        r[PH_COUNT] = phase_now();
     }
 }
 __device__ __forceinline__ void phase_mark(int k) {
-    const unsigned long long t = __builtin_amdgcn_s_memtime();
+    const unsigned long long t = phase_now();
     if (phase_leader()) {
         unsigned long long* r = phase_row[threadIdx.x >> 6];
         r[k] += t - r[PH_COUNT];
@@ -276,7 +284,7 @@ struct GView {
     uint32_t* A;             // [Rp] (entries R .. Rp-1 stay 0)
     int32_t* Kc;             // [Rp]
     uint16_t* Qx;            // [R] own queue region, or null: the queue lives in the trial state
-    uint32_t* F;             // [2S+1] reactant factors (the Kc region when it is large enough)
+    uint32_t* F;             // [2S+1] reactant factors, in the Kc region (used only when 2S + 1 <= Rp)
     uint32_t* qn;            // length of the Poisson queue (LDS atomic counter)
 };
 
@@ -296,8 +304,7 @@ __host__ __device__ __forceinline__ size_t group_lds_words(int S, int R, int G) 
     const size_t Rp = (size_t)group_rpad(R, G);
     const size_t q = ((size_t)R + 1) / 2;
     const size_t qx = q > (size_t)S ? q : 0;
-    const size_t fx = (size_t)2 * S + 1 > Rp ? (size_t)2 * S + 1 : 0;
-    return (size_t)2 * S + 2 * Rp + (qx > fx ? qx : fx) + 1;
+    return (size_t)2 * S + 2 * Rp + qx + 1;
 }
 
 __device__ __forceinline__ GView group_view(uint32_t* base, int S, int R, int G) {
@@ -307,7 +314,7 @@ __device__ __forceinline__ GView group_view(uint32_t* base, int S, int R, int G)
     V.A = (uint32_t*)(V.Xb + 2 * S);
     V.Kc = (int32_t*)(V.A + Rp);
     V.Qx = ((size_t)R + 1) / 2 > (size_t)S ? (uint16_t*)(V.Kc + Rp) : nullptr;
-    V.F = 2 * S + 1 <= Rp ? (uint32_t*)V.Kc : (uint32_t*)(V.Kc + Rp);
+    V.F = (uint32_t*)V.Kc;              // factor table (only when 2S + 1 <= Rp: see pfast)
     V.qn = base + group_lds_words(S, R, G) - 1;
     return V;
 }
@@ -1019,11 +1026,15 @@ __device__ __forceinline__ void gstage_get(const KParams& p, GLane& L, int s0, i
     }
 }
 
-// New region of this group after a stable partition of the workgroup's
-// regions by key (0 leap, 1 SSA, 2 none); cnt: 3 words per wavefront, perm:
-// one word per region.  Every lane of the workgroup must call it.
+// Stable partition of the workgroup's regions by key (0 leap, 1 SSA, 2
+// none); cnt: 3 words per wavefront, perm: one word per region.  Every lane
+// of the workgroup must call it.  Returns false, with no other effect, when
+// no region of the workgroup holds a simulation; else the new region of
+// this lane's group in *next and whether any region moves to other lanes in
+// *moves (the same for the whole workgroup).
 template <int G>
-__device__ __forceinline__ int gregroup(int key, int reg, uint32_t* cnt, uint32_t* perm, int l) {
+__device__ __forceinline__ bool gregroup(int key, int reg, uint32_t* cnt, uint32_t* perm, int l, int* next,
+                                         bool* moves) {
     const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     uint32_t rank = 0;
@@ -1034,13 +1045,20 @@ __device__ __forceinline__ int gregroup(int key, int reg, uint32_t* cnt, uint32_
         if (lane == 0) cnt[wave * 3 + k] = (uint32_t)__popcll(b);
     }
     __syncthreads();
-    uint32_t pos = 0;
+    uint32_t active = 0, pos = 0;
+    for (int w = 0; w < nw; ++w) active += cnt[w * 3] + cnt[w * 3 + 1];
+    if (active == 0) return false;
     for (int k = 0; k < key; ++k)
         for (int w = 0; w < nw; ++w) pos += cnt[w * 3 + k];
     for (int w = 0; w < wave; ++w) pos += cnt[w * 3 + key];
     if (l == 0) perm[pos + rank] = (uint32_t)reg;
     __syncthreads();
-    return (int)perm[threadIdx.x / G];
+    const int nreg = (int)(blockDim.x / G);
+    bool mv = false;
+    for (int q = 0; q < nreg; ++q) mv = mv || perm[q] != (uint32_t)q;
+    *moves = mv;
+    *next = (int)perm[threadIdx.x / G];
+    return true;
 }
 
 // Persistent group kernel: p.ns lanes = p.ns / G group slots, each pulling
@@ -1118,18 +1136,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G
             if (it % p.compact == 0) {
                 // 0: leap (or about to take a simulation), 1: SSA, 2: no simulation left
                 const int key = L.sim == -2 ? 2 : (L.sim >= 0 ? L.ssa_mode : kuse(p.force_ssa));
-                if (!__syncthreads_or(key != 2)) break;
+                int ng;
+                bool moves;
+                if (!gregroup<G>(key, gib, rg_cnt, rg_perm, l, &ng, &moves)) break;
                 // a group whose region stays on its lanes keeps its registers;
                 // the others hand their state over through the slot arrays
-                const int ng = gregroup<G>(key, gib, rg_cnt, rg_perm, l);
-                const bool move = ng != gib;
-                if (move) gstage_put<G>(p, L, L.slot, l);
-                __syncthreads();
-                if (move) {
-                    gib = ng;
-                    L.slot = (int)blockIdx.x * nreg + gib;
-                    gstage_get<G>(p, L, L.slot, l);
-                    V = group_view(smem + (size_t)gib * gw, p.S, p.R, G);
+                if (moves) {
+                    const bool move = ng != gib;
+                    if (move) gstage_put<G>(p, L, L.slot, l);
+                    __syncthreads();
+                    if (move) {
+                        gib = ng;
+                        L.slot = (int)blockIdx.x * nreg + gib;
+                        gstage_get<G>(p, L, L.slot, l);
+                        V = group_view(smem + (size_t)gib * gw, p.S, p.R, G);
+                    }
                 }
                 CTL_PHASE(PH_REGROUP);
             }
@@ -1246,7 +1267,12 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
         oe = (int)T.size();
         T.insert(T.end(), e.begin(), e.end());
     };
-    p.pfast = 1;
+    // the factor table lives in the Kc region, free outside the count and
+    // update phases; a model whose table would not fit there (2S + 1 > Rp)
+    // computes propensities from the state instead of enlarging every
+    // group's LDS (which would cost occupancy: the enzyme cascade, 61 species
+    // and 60 reactions at G = 4, ran 20 % slower with a separate table)
+    p.pfast = 2 * m.S + 1 <= group_rpad(m.R, G) ? 1 : 0;
     for (int j = 0; j < m.R; ++j) {
         if (m.Lptr[j + 1] - m.Lptr[j] > 2) p.pfast = 0;
         for (uint32_t h = m.Lptr[j]; h < m.Lptr[j + 1]; ++h)

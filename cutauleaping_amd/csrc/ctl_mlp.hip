@@ -57,6 +57,7 @@ struct GemmArgs {
     int a_vec, b_vec;          // 16-byte loads allowed (16-byte aligned base, ld a multiple of 8)
     int M, N, K;
     int tiles_n;               // column tiles
+    int kchunk;                // k per split (split-K: blockIdx.y selects [y*kchunk, (y+1)*kchunk))
     int epi;
     const float* bias;         // [N] (EPI_BIAS_*)
     const uint16_t* mask;      // [M][N] bf16 activation whose sign gates (EPI_MASK_BF16)
@@ -96,33 +97,54 @@ __device__ __forceinline__ uint4 gload8(const uint16_t* __restrict__ X, long ld,
 }
 
 // Rows [r0, r0 + BR) x k [k0, k0 + GBK) of an operand: BR*GBK/8 vectors,
-// BR/64 per lane.  Vector v covers (row v/4, k 8*(v%4)..) when kc, else
-// (rows 8*(v % (BR/8)).., k v / (BR/8)).
+// BR/64 per lane.  Vector v covers (row v/4, k 8*(v%4)..) when kc; else a
+// lane loads the two k rows of a pair, vectors v and v + 1 = (rows 8*u..,
+// k 2*(w%16) and 2*(w%16) + 1) with w = v/2, u = w/16, so that the store
+// can write each row's two k values as one 32-bit word and the 16 lanes of
+// a row block write 16 consecutive words.
 template <int BR>
 __device__ __forceinline__ void gstage_load(uint4 (&reg)[BR / 64], const uint16_t* __restrict__ X, long ld, bool kc,
                                             bool vec, int r0, int k0, int R, int K) {
 #pragma unroll
     for (int q = 0; q < BR / 64; ++q) {
-        const int v = (int)threadIdx.x + 256 * q;
-        if (kc)
+        const int v = (int)threadIdx.x * (BR / 64) + q;
+        if (kc) {
             reg[q] = gload8(X, ld, true, vec, r0 + (v >> 2), k0 + (v & 3) * 8, R, K);
-        else
+        } else if constexpr (BR / 64 >= 2) {
+            const int w = v >> 1, u = w >> 4;
+            reg[q] = gload8(X, ld, false, vec, r0 + u * 8, k0 + 2 * (w & 15) + (v & 1), R, K);
+        } else {                         // one vector per lane: (rows 8*(v % (BR/8)).., k v / (BR/8))
             reg[q] = gload8(X, ld, false, vec, r0 + (v % (BR / 8)) * 8, k0 + v / (BR / 8), R, K);
+        }
     }
 }
 
 template <int BR>
 __device__ __forceinline__ void gstage_store(const uint4 (&reg)[BR / 64], uint16_t* S, bool kc) {
+    if (kc) {
 #pragma unroll
-    for (int q = 0; q < BR / 64; ++q) {
-        const int v = (int)threadIdx.x + 256 * q;
-        if (kc) {
+        for (int q = 0; q < BR / 64; ++q) {
+            const int v = (int)threadIdx.x * (BR / 64) + q;
             *reinterpret_cast<uint4*>(S + (v >> 2) * GKP + (v & 3) * 8) = reg[q];
-        } else {
-            const int r = (v % (BR / 8)) * 8, k = v / (BR / 8);
-            const uint32_t w[4] = {reg[q].x, reg[q].y, reg[q].z, reg[q].w};
+        }
+    } else if constexpr (BR / 64 < 2) {
+        const int v = (int)threadIdx.x, r = (v % (BR / 8)) * 8, k = v / (BR / 8);
+        const uint32_t w[4] = {reg[0].x, reg[0].y, reg[0].z, reg[0].w};
 #pragma unroll
-            for (int e = 0; e < 8; ++e) S[(r + e) * GKP + k] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
+        for (int e = 0; e < 8; ++e) S[(r + e) * GKP + k] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
+    } else {
+#pragma unroll
+        for (int q = 0; q < BR / 64; q += 2) {
+            const int w = ((int)threadIdx.x * (BR / 64) + q) >> 1, u = w >> 4;
+            const uint32_t lo[4] = {reg[q].x, reg[q].y, reg[q].z, reg[q].w};          // k even
+            const uint32_t hi[4] = {reg[q + 1].x, reg[q + 1].y, reg[q + 1].z, reg[q + 1].w};   // k odd
+            uint32_t* row = reinterpret_cast<uint32_t*>(S + (u * 8) * GKP + 2 * (w & 15));
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t a = (lo[e >> 1] >> (16 * (e & 1))) & 0xffffu;
+                const uint32_t b = (hi[e >> 1] >> (16 * (e & 1))) & 0xffffu;
+                row[e * (GKP / 2)] = a | (b << 16);
+            }
         }
     }
 }
@@ -138,6 +160,8 @@ __global__ __launch_bounds__(256) void gemm_tile(GemmArgs g) {
     const int full = nb / 8 * 8;
     const int id = b < full ? (b % 8) * (full / 8) + b / 8 : b;
     const int m0 = (id / g.tiles_n) * BM, n0 = (id % g.tiles_n) * BM;
+    const int kb = (int)blockIdx.y * g.kchunk;                      // this split's k range [kb, ke)
+    const int ke = kb + g.kchunk < g.K ? kb + g.kchunk : g.K;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = (wave >> 1) * 32 * WT, wn = (wave & 1) * 32 * WT;
     f32x16 acc[WT][WT];
@@ -148,16 +172,16 @@ __global__ __launch_bounds__(256) void gemm_tile(GemmArgs g) {
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[a][c][v] = 0.0f;
     uint4 ra[BM / 64], rb[BM / 64];
-    gstage_load<BM>(ra, g.A, g.lda, g.a_kc, g.a_vec, m0, 0, g.M, g.K);
-    gstage_load<BM>(rb, g.B, g.ldb, g.b_kc, g.b_vec, n0, 0, g.N, g.K);
+    gstage_load<BM>(ra, g.A, g.lda, g.a_kc, g.a_vec, m0, kb, g.M, ke);
+    gstage_load<BM>(rb, g.B, g.ldb, g.b_kc, g.b_vec, n0, kb, g.N, ke);
     gstage_store<BM>(ra, As, g.a_kc);
     gstage_store<BM>(rb, Bs, g.b_kc);
     __syncthreads();
-    for (int k0 = 0; k0 < g.K; k0 += GBK) {
-        const bool more = k0 + GBK < g.K;
+    for (int k0 = kb; k0 < ke; k0 += GBK) {
+        const bool more = k0 + GBK < ke;
         if (more) {
-            gstage_load<BM>(ra, g.A, g.lda, g.a_kc, g.a_vec, m0, k0 + GBK, g.M, g.K);
-            gstage_load<BM>(rb, g.B, g.ldb, g.b_kc, g.b_vec, n0, k0 + GBK, g.N, g.K);
+            gstage_load<BM>(ra, g.A, g.lda, g.a_kc, g.a_vec, m0, k0 + GBK, g.M, ke);
+            gstage_load<BM>(rb, g.B, g.ldb, g.b_kc, g.b_vec, n0, k0 + GBK, g.N, ke);
         }
 #pragma unroll
         for (int ks = 0; ks < GBK; ks += 16) {
@@ -193,13 +217,22 @@ __global__ __launch_bounds__(256) void gemm_tile(GemmArgs g) {
                 const long o = (long)row * g.N + col;
                 float x = acc[a][c][v];
                 switch (g.epi) {
-                    case EPI_STORE_F32: g.Cf[o] = x; break;
+                    case EPI_STORE_F32: g.Cf[(long)blockIdx.y * g.M * g.N + o] = x; break;   // split y's partial
                     case EPI_BIAS_RELU_BF16: x += g.bias[col]; g.Cb[o] = f2bf(x > 0.0f ? x : 0.0f); break;
                     case EPI_BIAS_F32: g.Cf[o] = x + g.bias[col]; break;
                     case EPI_MASK_BF16: g.Cb[o] = f2bf(bf2f(g.mask[o]) > 0.0f ? x : 0.0f); break;
                 }
             }
         }
+}
+
+// Split-K partial sums, added in split order: C[i] = sum_s part[s][i].
+__global__ void ksplit_sum(const float* __restrict__ part, float* __restrict__ C, int splits, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int k = 0; k < splits; ++k) s += part[(long)k * n + i];
+    C[i] = s;
 }
 
 // Loss gradient of the mean squared error over [B][N]: d = 2 (y - t) / (B N),
@@ -228,12 +261,33 @@ __global__ void mse_grad(const float* __restrict__ y, const float* __restrict__ 
     }
 }
 
-// db[j] = sum_b d[b][j] (d in BF16), FP32.
-__global__ void col_sum(const uint16_t* __restrict__ d, float* __restrict__ db, int B, int N) {
+// Column sums db[j] = sum_b d[b][j] (d in BF16), FP32, in two passes with a
+// fixed order: col_sum_part sums rows [c*CS_ROWS, (c+1)*CS_ROWS) of 64
+// columns per workgroup (4 row lanes per column, combined in LDS), col_sum
+// adds the chunks in order.  Coalesced: a wavefront reads 64 consecutive
+// columns of one row.
+constexpr int CS_ROWS = 512;
+__global__ __launch_bounds__(256) void col_sum_part(const uint16_t* __restrict__ d, float* __restrict__ part, int B,
+                                                    int N) {
+    __shared__ float red[4][64];
+    const int j = blockIdx.x * 64 + (int)(threadIdx.x & 63u);
+    const int q = (int)(threadIdx.x >> 6);
+    const int b0 = blockIdx.y * CS_ROWS;
+    const int b1 = b0 + CS_ROWS < B ? b0 + CS_ROWS : B;
+    float s = 0.0f;
+    if (j < N)
+        for (int b = b0 + q; b < b1; b += 4) s += bf2f(d[(long)b * N + j]);
+    red[q][threadIdx.x & 63u] = s;
+    __syncthreads();
+    if (q == 0 && j < N)
+        part[(long)blockIdx.y * N + j] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+__global__ void col_sum(const float* __restrict__ part, float* __restrict__ db, int chunks, int N) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
     float s = 0.0f;
-    for (int b = 0; b < B; ++b) s += bf2f(d[(long)b * N + j]);
+    for (int c = 0; c < chunks; ++c) s += part[(long)c * N + j];
     db[j] = s;
 }
 
@@ -292,6 +346,9 @@ struct ctl_mlp {
     float* part = nullptr;
     int32_t* idx = nullptr;             // [ndata] epoch permutation
     double* lacc = nullptr;             // epoch loss total
+    float* cpart = nullptr;             // [ceil(cap / CS_ROWS)][max w] column-sum partials
+    float* kpart = nullptr;             // split-K partial tiles of the weight gradients
+    long kpart_words = 0;
     // training data
     float *dX = nullptr, *dY = nullptr;
     long ndata = 0;
@@ -337,16 +394,34 @@ void gemm(ctl_mlp* n, const uint16_t* A, long ai, long ak, const uint16_t* B, lo
     g.b_vec = (g.ldb % 8 == 0) && ((uintptr_t)B % 16 == 0);
     g.M = M; g.N = N; g.K = K;
     g.epi = epi; g.bias = bias; g.mask = mask; g.Cf = Cf; g.Cb = Cb;
+    g.kchunk = K;
     // 128 x 128 tiles when they still give every CU a workgroup, else 64 x 64
     const long big = (long)((M + 127) / 128) * ((N + 127) / 128);
     if (big >= 256) {
         g.tiles_n = (N + 127) / 128;
         hipLaunchKernelGGL(gemm_tile<2>, dim3((unsigned)big), dim3(256), 0, n->stream, g);
-    } else {
-        g.tiles_n = (N + 63) / 64;
-        const long tiles = (long)((M + 63) / 64) * g.tiles_n;
-        hipLaunchKernelGGL(gemm_tile<1>, dim3((unsigned)tiles), dim3(256), 0, n->stream, g);
+        return;
     }
+    g.tiles_n = (N + 63) / 64;
+    const long tiles = (long)((M + 63) / 64) * g.tiles_n;
+    // a long k loop over few tiles (weight gradients over a large batch):
+    // split k so that about 1024 workgroups run, partial tiles summed in order
+    int splits = 1;
+    if (epi == EPI_STORE_F32 && n->kpart)
+        while (splits < 64 && tiles * splits * 2 <= 1024 && K / (splits * 2) >= 1024 &&
+               (long)splits * 2 * M * N <= n->kpart_words)
+            splits *= 2;
+    if (splits == 1) {
+        hipLaunchKernelGGL(gemm_tile<1>, dim3((unsigned)tiles), dim3(256), 0, n->stream, g);
+        return;
+    }
+    g.kchunk = (K + splits - 1) / splits;
+    g.kchunk = (g.kchunk + GBK - 1) / GBK * GBK;
+    g.Cf = n->kpart;
+    const long mn = (long)M * N;
+    hipLaunchKernelGGL(gemm_tile<1>, dim3((unsigned)tiles, (unsigned)splits), dim3(256), 0, n->stream, g);
+    hipLaunchKernelGGL(ksplit_sum, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, n->stream, n->kpart, Cf, splits,
+                       mn);
 }
 
 int ensure_cap(ctl_mlp* n, int B) {
@@ -360,14 +435,21 @@ int ensure_cap(ctl_mlp* n, int B) {
     if ((rc = malloc_dev(n, &n->err0, (size_t)B * wmax)) || (rc = malloc_dev(n, &n->err1, (size_t)B * wmax)) ||
         (rc = malloc_dev(n, &n->yout, (size_t)B * n->widths[n->L])) ||
         (rc = malloc_dev(n, &n->tgt, (size_t)B * n->widths[n->L])) || (rc = malloc_dev(n, &n->part, 1024)) ||
-        (rc = malloc_dev(n, &n->lacc, 1)))
+        (rc = malloc_dev(n, &n->lacc, 1)) ||
+        (rc = malloc_dev(n, &n->cpart, (size_t)((B + CS_ROWS - 1) / CS_ROWS) * wmax)))
         return rc;
+    if (B >= 2048) {                    // large batches: room for 16 splits of the largest weight block
+        long wmn = 0;
+        for (int l = 0; l < n->L; ++l) wmn = std::max(wmn, (long)n->widths[l] * n->widths[l + 1]);
+        if ((rc = malloc_dev(n, &n->kpart, (size_t)16 * wmn))) return rc;
+        n->kpart_words = 16 * wmn;
+    }
     n->cap = B;
     return 0;
 }
 
-// Forward pass of B rows already in act[0]; output FP32 in yout.
-void forward(ctl_mlp* n, int B) {
+// Forward pass of B rows already in act[0]; output FP32 in out (default yout).
+void forward(ctl_mlp* n, int B, float* out = nullptr) {
     for (int l = 0; l < n->L; ++l) {
         const int K = n->widths[l], N = n->widths[l + 1];
         const uint16_t* W = n->wb + n->woff[l];
@@ -375,7 +457,7 @@ void forward(ctl_mlp* n, int B) {
         if (l + 1 < n->L)
             gemm(n, n->act[l], K, 1, W, N, 1, B, N, K, EPI_BIAS_RELU_BF16, b, nullptr, nullptr, n->act[l + 1]);
         else
-            gemm(n, n->act[l], K, 1, W, N, 1, B, N, K, EPI_BIAS_F32, b, nullptr, n->yout, nullptr);
+            gemm(n, n->act[l], K, 1, W, N, 1, B, N, K, EPI_BIAS_F32, b, nullptr, out ? out : n->yout, nullptr);
     }
 }
 
@@ -387,7 +469,10 @@ void backward(ctl_mlp* n, int B) {
         const int K = n->widths[l], N = n->widths[l + 1];
         // dW_l = act[l]^T * e : (K x B) * (B x N)
         gemm(n, n->act[l], 1, K, e, N, 1, K, N, B, EPI_STORE_F32, nullptr, nullptr, n->g + n->woff[l], nullptr);
-        hipLaunchKernelGGL(col_sum, dim3((N + 255) / 256), dim3(256), 0, n->stream, e, n->g + n->boff[l], B, N);
+        const int chunks = (B + CS_ROWS - 1) / CS_ROWS;
+        hipLaunchKernelGGL(col_sum_part, dim3((N + 63) / 64, chunks), dim3(256), 0, n->stream, e, n->cpart, B, N);
+        hipLaunchKernelGGL(col_sum, dim3((N + 255) / 256), dim3(256), 0, n->stream, n->cpart, n->g + n->boff[l],
+                           chunks, N);
         if (l > 0) {
             // e_{l} = (e * W_l^T) gated by act[l] > 0 : (B x N) * (N x K)
             const uint16_t* W = n->wb + n->woff[l];
@@ -531,10 +616,7 @@ int ctl_mlp_forward_device(ctl_mlp* n, const float* x_dev, int64_t rows, float* 
         const long nin = (long)B * w0;
         hipLaunchKernelGGL(to_bf16, dim3((unsigned)((nin + 255) / 256)), dim3(256), 0, n->stream, x_dev + r0 * w0,
                            n->act[0], nin);
-        forward(n, B);
-        hipError_t e = hipMemcpyAsync(y_dev + r0 * wL, n->yout, sizeof(float) * (size_t)B * wL,
-                                      hipMemcpyDeviceToDevice, n->stream);
-        if (e != hipSuccess) return mfail(e, "D2D");
+        forward(n, B, y_dev + r0 * wL);              // the output layer writes the caller's rows
     }
     hipError_t e = hipSuccess;
     if (ms) e = hipEventRecord(e1, n->stream);

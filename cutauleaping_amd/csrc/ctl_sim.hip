@@ -502,7 +502,20 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     if (G > 1) {
         const size_t gwords = group_lds_words(S, R, G);    // group LDS words per simulation
         if (gwords * 4 > 65536) return bail(ctl::fail(CTL_E_LIMIT, "model too large for the group kernel (more than 64 KB of LDS)"));
-        int gpb = 256 / G;                                    // groups per workgroup
+        // regrouping simulations by mode (leap / SSA) every `compact` iterations;
+        // by default for groups of 16 or more lanes (networks of more than 128
+        // reactions, where a leap costs several SSA steps) and never in
+        // force-SSA mode, where every simulation is in one mode (measured: the
+        // 100x200 network, G = 16: +10 %; the 50x100 sweep network, G = 8:
+        // -6 %; the SSA-dominated enzyme cascade, G = 4: -40 %)
+        const int compact = o->compact_interval > 0 ? o->compact_interval
+                            : (o->compact_interval < 0 || o->force_ssa || G < 16) ? 0 : 8;
+        // groups per workgroup: block_size / G when given (64..256), else two
+        // wavefronts when regrouping (fewer wavefronts wait for the slowest at
+        // each regrouping: C1 +1.5 % over four) and four otherwise
+        if (o->block_size > 0 && (o->block_size % 64 != 0 || o->block_size > 256))
+            return bail(ctl::fail(CTL_E_ARGS, "block_size must be 64..256, multiple of 64"));
+        int gpb = (o->block_size > 0 ? o->block_size : (compact > 0 && G < 64 ? 128 : 256)) / G;
         while (gpb > 64 / G && (size_t)gpb * gwords * 4 > 65536) gpb /= 2;
         if ((size_t)gpb * gwords * 4 > 65536) gpb = 64 / G;   // one wavefront
         if ((size_t)gpb * gwords * 4 > 65536)
@@ -510,8 +523,6 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
         s->block = gpb * G;
         s->smem = (size_t)gpb * gwords * 4;
         s->lds = true;
-        // regrouping simulations by mode (leap / SSA) every `compact` iterations
-        const int compact = o->compact_interval > 0 ? o->compact_interval : (o->compact_interval < 0 ? 0 : 8);
         s->compact = compact > 0 && G < 64 && s->block > 64 ? compact : 0;
         const void* kfn = group_kernel_fn(G, s->compact > 0);
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, s->block, s->smem);

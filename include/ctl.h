@@ -76,7 +76,8 @@ typedef struct {
     uint64_t seed;             /* Philox key (the reference used time(NULL), K.cu:176) */
     int32_t  verbose;
     /* launch geometry (0 = chosen by the library) */
-    int32_t  block_size;       /* threads per workgroup */
+    int32_t  block_size;       /* threads per workgroup (64..256, multiple of 64; 0: library default,
+                                  256 where LDS allows) */
     int32_t  slots;            /* resident simulation slots */
     int32_t  iters_per_launch; /* leap/SSA iterations per slot per launch */
     int32_t  ssa_chunk;        /* SSA steps per iteration of a slot (0: 10 for one lane per
@@ -88,8 +89,8 @@ typedef struct {
                                   regroups its simulations so that leap-mode and SSA-mode ones run
                                   in different wavefronts, SSA bursts then running ssa_chunk steps
                                   per iteration; trajectories do not depend on it.  0: library
-                                  default (8 for 2..32 lanes per simulation, never for one lane),
-                                  < 0: never */
+                                  default (8 for 16 or 32 lanes per simulation outside
+                                  force-SSA mode, else never), < 0: never */
 } ctl_opts;
 
 typedef struct {

@@ -256,27 +256,32 @@ def test_autotune_batch_classes(gpu, oracle_lib):
         m.close()
 
 
-@pytest.mark.parametrize("seed,G,compact,chunk,iters,slots,force", [
-    (40, 2, 1, 1, 5, 0, False),
-    (42, 4, 3, 7, 11, 0, True),
-    (43, 8, 8, 6, 0, 0, False),
-    (49, 16, 2, 3, 9, 24, False),
-    (47, 32, 5, 100, 13, 0, False),
-    (40, 16, -1, 0, 0, 0, False),
+@pytest.mark.parametrize("seed,G,compact,chunk,iters,slots,force,block", [
+    (40, 2, 1, 1, 5, 0, False, 0),
+    (42, 4, 3, 7, 11, 0, True, 0),
+    (43, 8, 8, 6, 0, 0, False, 0),
+    (49, 16, 2, 3, 9, 24, False, 0),
+    (47, 32, 5, 100, 13, 0, False, 0),
+    (40, 16, -1, 0, 0, 0, False, 0),
+    (43, 16, 8, 6, 5, 0, False, 128),
+    (49, 8, 4, 5, 7, 0, False, 64),
 ])
-def test_group_regrouping_bit_exact(gpu, oracle_lib, seed, G, compact, chunk, iters, slots, force):
+def test_group_regrouping_bit_exact(gpu, oracle_lib, seed, G, compact, chunk, iters, slots, force, block):
     """A workgroup's simulations are reassigned to its wavefronts by mode
     (leap / SSA) every `compact` iterations, SSA bursts running `chunk` steps
     per iteration; with small iteration budgets and slot counts that park
-    simulations (and half-run bursts) across launches, every sample and
-    counter equals the oracle's, which knows no scheduling."""
+    simulations (and half-run bursts) across launches, and workgroups of
+    four, two or one wavefront (one: burst pairing instead), every sample
+    and counter equals the oracle's, which knows no scheduling."""
     print(f"case: group regrouping seed={seed} G={G} compact={compact} chunk={chunk} iters={iters} "
-          f"slots={slots} force={force}", flush=True)
+          f"slots={slots} force={force} block={block}", flush=True)
     net, feed, x0, c, cs, ts = _random_case(seed, 14, 30, 200, True)
     m = gpu.Model.from_arrays(net["left"], net["right"], x0, c, ts, feed=feed, cs=cs)
     with gpu.Sim(m, 200, 1, seed=seed, lanes_per_sim=G, force_ssa=force, compact_interval=compact,
-                 ssa_chunk=chunk, iters_per_launch=iters, slots=slots) as s:
+                 ssa_chunk=chunk, iters_per_launch=iters, slots=slots, block_size=block) as s:
         s.run()
+        if block:
+            assert s.stats()["block_size"] == block
         out, st, xf = oracle_lib.run(net["left"], net["right"], x0, c, ts, locked=feed, cs=cs, n=200,
                                      seed=seed, force_ssa=force, final=True, group=G)
         if not force:       # both modes present, so regrouping has something to separate
