@@ -62,6 +62,21 @@ def lane_ops(stats, S, R, nL, nV, nF, SV):
     return float(w)
 
 
+def model_rooflines(model, stats, kernel_ms, nsims):
+    """VALU and HBM fractions of a single-model run: algorithmic lane-ops
+    (lane_ops) and bytes (inputs, samples and per-simulation record) over the
+    kernel time."""
+    info = model.info()
+    sd = model.structure()
+    SV = int(len(np.unique(sd["rav_flat"][:, 1]))) if info.nV else 0
+    work = lane_ops(stats, info.S, info.R, info.nL, info.nV, info.nF, SV)
+    secs = kernel_ms / 1e3
+    hbm = nsims * (4 * info.S + 4 * info.SS * info.K + 64)
+    return {"valu": {"achieved": work / secs / 1e12, "peak": VALU_PEAK / 1e12, "unit": "Tlane-op/s",
+                     "frac": work / secs / VALU_PEAK},
+            "hbm": {"achieved_GBps": hbm / secs / 1e9, "peak_GBps": HBM_PEAK / 1e9, "frac": hbm / secs / HBM_PEAK}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,10 +97,10 @@ def main():
     ap.add_argument("--compact", type=int, default=0,
                     help="iterations between regroupings of simulations by mode (0: library default, <0: off)")
     ap.add_argument("--c0-sims", type=int, default=1024, help="c0: simulations (the configuration runs 1024)")
-    ap.add_argument("--workload", default="c1", choices=("c0", "c1", "c2", "c3", "c4", "mlp"),
-                    help="c1: the headline (default); c0: Schlogl; c2: batched uneven models; "
-                         "c3: surrogate-screened sweep; c4: enzyme cascade (adaptive and force-SSA); "
-                         "mlp: the surrogate network's kernels alone")
+    ap.add_argument("--workload", default="c1", choices=("c0", "c1", "c1ref", "c2", "c3", "c4", "mlp"),
+                    help="c1: the headline (default); c1ref: C1's first slice through the reference's execution "
+                         "design; c0: Schlogl; c2: batched uneven models; c3: surrogate-screened sweep; "
+                         "c4: enzyme cascade (adaptive and force-SSA); mlp: the surrogate network's kernels alone")
     ap.add_argument("--mlp-rows", type=int, default=65536, help="mlp: rows (parameter sets) per pass")
     ap.add_argument("--models", type=int, default=8192, help="c2: models in the batch")
     ap.add_argument("--no-tune", action="store_true", help="c2: the library's size rule instead of the autotuner")
@@ -111,7 +126,8 @@ def main():
             args.steps = 1
         if "--warmup" not in sys.argv:
             args.warmup = 0
-        {"c0": bench_c0, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "mlp": bench_mlp}[args.workload](args)
+        {"c0": bench_c0, "c1ref": bench_c1ref, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4,
+         "mlp": bench_mlp}[args.workload](args)
         return
 
     import cutauleaping_amd as ctl
@@ -341,7 +357,8 @@ def bench_c0(args):
         dt = time.perf_counter() - t1
     extra = {"simulations_per_s": n * steps / dt, "kernel_ms_per_step": kms / steps,
              "counters": {k: st[k] for k in ("leaps", "ssa_steps", "switches", "poisson", "gaps", "launches",
-                                             "slots", "block_size", "lds_state", "lanes_per_sim")}}
+                                             "slots", "block_size", "lds_state", "lanes_per_sim")},
+             "roofline": model_rooflines(model, st, kms / steps, n)}
     if not args.no_cpu:
         extra["cpu_baseline"] = cpu_baseline(net, ts, min(n, 4096), locked=[1, 0, 1])
     _line("reaction firings/sec on 1 GPU (Schlogl)", fir / dt, "firings/s", steps, dt * 1e3 / steps, "f32",
@@ -398,6 +415,9 @@ def bench_c2(args):
             print(f"# step {k}: firings={st['firings']} kernel_ms={st['kernel_ms']:.1f} launches={st['launches']}",
                   file=sys.stderr, flush=True)
     sims = sum(reps)
+    extra = {}
+    if not args.no_cpu:
+        extra["cpu_baseline"] = c2_cpu_baseline(nets, ts)
     _line("reaction firings/sec on 1 GPU (batched uneven models)", fir / (ms / 1e3), "firings/s", args.steps,
           ms / args.steps, "f32",
           {"workload": f"{len(nets)} mass-action models, N 10..500, M 10..1000, {sims} simulations "
@@ -411,15 +431,80 @@ def bench_c2(args):
           simulations_per_s=sims * args.steps / (ms / 1e3),
           counters={k: st[k] for k in ("leaps", "ssa_steps", "switches", "poisson", "gaps", "launches",
                                        "slots", "lanes_per_sim")},
-          kernel_ms=st["kernel_ms"])
+          kernel_ms=st["kernel_ms"], **extra)
     for m in models:
         m.close()
+
+
+def c2_cpu_baseline(nets, ts, every=512, per_model=16):
+    """The oracle's REFSEM restatement over a sample of the batch: every
+    512th model, its first min(replicas, 16) simulations, OpenMP over the
+    simulations of a model on the host cores (about 20 s: the reference
+    algorithm recomputes all propensities at every SSA step, which the
+    large models of the batch make slow)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    import oracle
+    threads, host = host_cores()
+    fir, nsim, secs = 0, 0, 0.0
+    for n in nets[::every]:
+        k = min(int(n["replicas"]), per_model)
+        t0 = time.perf_counter()
+        _, st = oracle.run(n["left"], n["right"], n["x0"], n["c"], ts, cs=n["cs"], n=k, seed=0xC0FFEE,
+                           precision=oracle.REFSEM, threads=threads)
+        secs += time.perf_counter() - t0
+        fir += int(st["firings"].sum())
+        nsim += k
+    return {"value": fir / secs, "unit": "firings/s", "cores": threads, "kind": "port",
+            "sample": f"every {every}th model of the batch ({len(nets[::every])} models), the first "
+                      f"min(replicas, {per_model}) simulations of each ({nsim} simulations, {fir} firings, "
+                      f"{secs:.1f}s)", "simulations_per_s": nsim / secs, "host": host}
+
+
+def bench_c1ref(args):
+    """A denominator shaped like the reference kernel: the first 65536
+    simulations of the C1 workload through the reference's execution design,
+    restated in this build -- one lane per simulation with its state in HBM,
+    and one launch per step (a leap attempt, or a 100-step SSA burst for the
+    lanes that switched) followed by a blocking 4-byte copy of the done
+    counter (the reference runs three launches and that copy per step,
+    kernel.cu:242-345).  It computes in FP32 where the reference computes
+    propensities and tau in FP64, so it is faster than the reference kernel
+    would be on this GPU.  A step is one complete run of the 65536."""
+    import cutauleaping_amd as ctl
+    from cutauleaping_amd import networks
+    net = networks.mass_action(100, 200, seed=args.seed, t_end=50.0)
+    ts = networks.sample_times(200, 50.0)
+    model = ctl.Model.from_arrays(net["left"], net["right"], net["x0"], net["c"], ts)
+    n = 65536
+    fir, ms = 0, 0.0
+    steps = max(args.steps, 1)
+    with ctl.Sim(model, 256, n // 256, seed=0xC0FFEE, lanes_per_sim=1, iters_per_launch=1, ssa_chunk=100,
+                 compact_interval=-1, block_size=-1, verbose=True) as s:      # a line per launch on stderr
+        for k in range(steps):
+            t1 = time.perf_counter()
+            s.run()
+            ms += (time.perf_counter() - t1) * 1e3
+            st = s.stats()
+            fir += st["firings"]
+            print(f"# step {k}: firings={st['firings']} launches={st['launches']} kernel_ms={st['kernel_ms']:.1f}",
+                  file=sys.stderr, flush=True)
+    _line("reaction firings/sec on 1 GPU (reference-shaped execution)", fir / (ms / 1e3), "firings/s", steps,
+          ms / steps, "f32",
+          {"workload": "mass-action N=100 M=200 seed=1, the first 65536 simulations of the 2^20 workload, t_end=50; "
+                       "one lane per simulation, state in HBM, one launch per leap attempt / 100-step SSA burst "
+                       "with a blocking copy of the done counter after each", "simulations": n},
+          simulations_per_s=n * steps / (ms / 1e3),
+          counters={k: st[k] for k in ("leaps", "ssa_steps", "switches", "launches", "slots", "block_size",
+                                       "lds_state", "lanes_per_sim")},
+          kernel_ms=st["kernel_ms"], roofline=model_rooflines(model, st, st["kernel_ms"], n))
+    model.close()
 
 
 def bench_c4(args):
     """Enzyme cascade: networks.enzyme_cascade(20, seed=4) (61 species, 60
     reactions), 2^18 simulations, 100 samples to t_end=100, adaptive and
-    force-SSA (two lines)."""
+    force-SSA (two lines); at least one untimed run before the timed ones."""
     import cutauleaping_amd as ctl
     from cutauleaping_amd import networks
     net = networks.enzyme_cascade(20, seed=4)
@@ -431,12 +516,17 @@ def bench_c4(args):
         fir, ms = 0, 0.0
         with ctl.Sim(model, 256, n // 256, seed=0xC0FFEE, force_ssa=force, lanes_per_sim=args.lanes,
                      verbose=True) as s:
+            for k in range(max(args.warmup, 1)):      # untimed: the first run also loads the code object
+                s.run()
             for k in range(args.steps):
                 t1 = time.perf_counter()
                 s.run()
                 ms += (time.perf_counter() - t1) * 1e3
                 st = s.stats()
                 fir += st["firings"]
+        extra = {}
+        if not args.no_cpu:
+            extra["cpu_baseline"] = cpu_baseline(net, ts, min(n, args.cpu_sims), force_ssa=force)
         _line("reaction firings/sec on 1 GPU (enzyme cascade)", fir / (ms / 1e3), "firings/s", args.steps,
               ms / args.steps, "f32",
               {"workload": f"enzyme cascade 20 stages (61 species, 60 reactions), {n} simulations, 100 samples "
@@ -444,7 +534,7 @@ def bench_c4(args):
               simulations_per_s=n * args.steps / (ms / 1e3),
               counters={k: st[k] for k in ("leaps", "ssa_steps", "switches", "poisson", "gaps", "launches",
                                            "slots", "lanes_per_sim")},
-              kernel_ms=st["kernel_ms"])
+              kernel_ms=st["kernel_ms"], roofline=model_rooflines(model, st, st["kernel_ms"], n), **extra)
     model.close()
 
 

@@ -119,6 +119,36 @@ __device__ __forceinline__ void phase_flush() {
 #define CTL_PHASE_FLUSH() ((void)0)
 #endif
 
+// How propensities are computed (KParams::pfast): 0 from the reactant
+// entries (any multiplicity), 1 as two entries of the LDS factor table, 2 as
+// two entries computed from the state (models with at most two reactant
+// entries of multiplicity <= 2, 1 when the table fits in the Kc region).
+// PFM 0..2 fixes the mode at compile time (the simulation kernels: one
+// instance per mode, so none keeps another's registers live); PFM 3 reads
+// p.pfast (the batch kernels, whose models differ).
+template <int PFM>
+__device__ __forceinline__ int pmode(int pfast) {
+    if constexpr (PFM == 3) return pfast;
+    else return PFM;
+}
+
+// Two reactant entries (multiplicity 0, 1 or 2) from the state: the
+// arithmetic and the criticality test of greactant, in the same order.
+__device__ __forceinline__ void gtwo(float& a, bool& c, int32_t x0, uint32_t v0, int32_t x1, uint32_t v1,
+                                     uint32_t nc) {
+    const int32_t xs[2] = {x0, x1};
+    const uint32_t vs[2] = {v0, v1};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int32_t x = xs[k];
+        const uint32_t v = vs[k];
+        const float xf = (float)x;
+        const float f2 = (x < 2) ? 0.0f : (0.5f * xf) * (float)(x - 1);
+        a = a * (v == 1u ? xf : (v == 2u ? f2 : 1.0f));
+        c = c || ((uint32_t)x >= v && (uint32_t)x < nc * v);
+    }
+}
+
 // A kernel parameter read at its point of use: the compiler may not hoist a
 // test of it out of the main loop, where it would hold a 64-bit lane mask in
 // scalar registers for the whole kernel (and spill others).
@@ -382,13 +412,54 @@ __device__ __forceinline__ float gprop(const KParams& p, const int32_t* X, const
 // With pfast the two entries of a reaction are indices into the factor table
 // F of the state (built by gfactors), so a reaction costs two multiplies:
 // the same factors in the same order as greactant, hence the same bits.
-template <int G>
+template <int G, int PFM = 3>
 __device__ __forceinline__ void gprops(const KParams& p, const int32_t* X, const uint32_t* F, const float* crate,
                                        uint32_t* A, int l, float& P, float& PC) {
     const int nr = (p.R + G - 1) / G;
     P = 0.0f;
     PC = 0.0f;
-    if (kuse(p.pfast)) {
+    const int pm = pmode<PFM>(kuse(p.pfast));
+    if (pm == 2) {
+        // two entries per reaction (species | multiplicity << 16) read from
+        // the state; four rounds per batch of loads
+        const uint32_t* E = p.T + p.oPRe + l;
+        const uint32_t nc = (uint32_t)p.nc;
+        for (int r0 = 0; r0 < nr; r0 += 4) {
+            float cr[4];
+            uint32_t e0[4], e1[4];
+            int32_t x0[4], x1[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = r0 + q, j = l + r * G;
+                cr[q] = 0.0f;
+                e0[q] = e1[q] = 0u;
+                if (r < nr) {
+                    e0[q] = E[(2 * r) * G];
+                    e1[q] = E[(2 * r + 1) * G];
+                    if (j < p.R) cr[q] = crate[j];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                x0[q] = X[e0[q] & 0xffffu];
+                x1[q] = X[e1[q] & 0xffffu];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = r0 + q, j = l + r * G;
+                if (r < nr) {                          // past R: 0 (rate 0) into A's zero padding
+                    float a = cr[q];
+                    bool c = false;
+                    gtwo(a, c, x0[q], e0[q] >> 16, x1[q], e1[q] >> 16, nc);
+                    A[j] = __float_as_uint(a) | (c ? 0x80000000u : 0u);
+                    P = P + a;
+                    if (c) PC = PC + a;
+                }
+            }
+        }
+        return;
+    }
+    if (pm == 1) {
         // two entries per reaction; four rounds per batch of loads
         const uint32_t* E = p.T + p.oPRe + l;
         for (int r0 = 0; r0 < nr; r0 += 4) {
@@ -538,18 +609,18 @@ __device__ __forceinline__ void gabsorb(const KParams& p, GLane& L, const int32_
 }
 
 // One adaptive leap attempt (PreCalculations, T.cu:1568-2093).
-template <int G>
+template <int G, int PFM = 3>
 __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V, int l,
                                      const float* __restrict__ recip) {
     if (L.t >= p.tmax) return LEAP_DONE;
     int32_t* X = V.Xb + (L.xb ? p.S : 0);
     int32_t* X1 = V.Xb + (L.xb ? 0 : p.S);
     float P, PC;
-    if (kuse(p.pfast)) {
+    if (pmode<PFM>(kuse(p.pfast)) == 1) {
         gfactors<G>(p, X, V.F, l);
         gsync();
     }
-    gprops<G>(p, X, V.F, L.crate, V.A, l, P, PC);
+    gprops<G, PFM>(p, X, V.F, L.crate, V.A, l, P, PC);
     gsync();
     P = gscan<G>(P, l);
     PC = gscan<G>(PC, l);
@@ -785,19 +856,19 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
 // group's LDS), so the steps and their draws are those of one call.  After
 // the fired reaction only its dependent propensities are recomputed (same
 // values as recomputing all).  Returns true on absorption.
-template <int G>
+template <int G, int PFM = 3>
 __device__ __forceinline__ bool gssa_run(const KParams& p, GLane& L, const GView& V, int l, int nmax) {
     int32_t* X = V.Xb + (L.xb ? p.S : 0);
     if (L.ssa_left == 0) {
         rng_event(L.gg, L.evc++);
         L.tacc = 0.0;
         L.ssa_left = p.ssa_steps;
-        if (kuse(p.pfast)) {
+        if (pmode<PFM>(kuse(p.pfast)) == 1) {
             gfactors<G>(p, X, V.F, l);
             gsync();
         }
         float P, PC;
-        gprops<G>(p, X, V.F, L.crate, V.A, l, P, PC);
+        gprops<G, PFM>(p, X, V.F, L.crate, V.A, l, P, PC);
         gsync();
     }
     CTL_PHASE(PH_SSA_OPEN);
@@ -835,7 +906,8 @@ __device__ __forceinline__ bool gssa_run(const KParams& p, GLane& L, const GView
         }
         gsync();
         const uint32_t f1 = p.T[p.oFp + j + 1];
-        const bool fac = kuse(p.pfast) != 0;
+        const int pm = pmode<PFM>(kuse(p.pfast));
+        const bool fac = pm == 1;
         for (uint32_t h = p.T[p.oFp + j] + (uint32_t)l; h < f1; h += G) {
             const uint32_t ent = p.T[p.oFe + h];
             const int s = (int)(ent & 0xffffu);
@@ -861,6 +933,18 @@ __device__ __forceinline__ bool gssa_run(const KParams& p, GLane& L, const GView
                 a = a * __uint_as_float(u1 & 0x7fffffffu);
                 V.A[r] = __float_as_uint(a) | ((u0 | u1) & 0x80000000u);
             }
+        } else if (pm == 2) {
+            // dependent records {r, reactant entry 0, entry 1, 0} read from the state
+            const uint4* DR = reinterpret_cast<const uint4*>(p.T + p.oDR);
+            const uint32_t nc = (uint32_t)p.nc;
+            for (uint32_t d = d0 + (uint32_t)l; d < d1; d += G) {
+                const uint4 rec = DR[d];
+                const int r = (int)rec.x;
+                float a = L.crate[r];
+                bool c = false;
+                gtwo(a, c, X[rec.y & 0xffffu], rec.y >> 16, X[rec.z & 0xffffu], rec.z >> 16, nc);
+                V.A[r] = __float_as_uint(a) | (c ? 0x80000000u : 0u);
+            }
         } else {
             for (uint32_t d = d0 + (uint32_t)l; d < d1; d += G) {
                 const int r = (int)p.T[p.oDe + d];
@@ -882,10 +966,10 @@ __device__ __forceinline__ bool gssa_run(const KParams& p, GLane& L, const GView
 }
 
 // One whole SSA burst.
-template <int G>
+template <int G, int PFM = 3>
 __device__ __forceinline__ bool gssa(const KParams& p, GLane& L, const GView& V, int l) {
     L.ssa_left = 0;
-    return gssa_run<G>(p, L, V, l, p.ssa_steps);
+    return gssa_run<G, PFM>(p, L, V, l, p.ssa_steps);
 }
 
 __device__ __forceinline__ const float* grates(const KParams& p, int sim) {
@@ -1074,7 +1158,7 @@ __device__ __forceinline__ bool gregroup(int key, int reg, uint32_t* cnt, uint32
 // bytes of scratch): measured +9 % at G = 16 and +4 % at G = 32 on the
 // 100x200 network, +24 % at G = 8 on the 50x100 sweep network.  G = 4
 // groups are bounded by LDS instead, and G = 64 would spill heavily.
-template <int G, bool RG>
+template <int G, bool RG, int PF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G == 16 || G == 32 ? 4 : 1))) void group_kernel(KParams p) {
     extern __shared__ uint32_t smem[];
     __shared__ float recip[64];
@@ -1160,14 +1244,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G
                 CTL_PHASE(PH_PRE_SSA);
                 if (kuse(p.force_ssa) && L.ssa_left == 0 && L.t >= p.tmax) {
                     gfinish<G>(p, L, V, l);
-                } else if (gssa_run<G>(p, L, V, l, p.ssa_chunk)) {
+                } else if (gssa_run<G, PF>(p, L, V, l, p.ssa_chunk)) {
                     gfinish<G>(p, L, V, l);
                 } else if (L.ssa_left == 0 && !kuse(p.force_ssa)) {
                     L.ssa_mode = 0;
                 }
             } else {
                 CTL_PHASE(PH_PRE_LEAP);
-                const int r = gleap<G>(p, L, V, l, recip);
+                const int r = gleap<G, PF>(p, L, V, l, recip);
                 if (r == LEAP_DONE || r == LEAP_ABSORB) {
                     gfinish<G>(p, L, V, l);
                 } else if (r == LEAP_SWITCH) {
@@ -1178,12 +1262,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G
         }
         if (L.sim < 0 && !gfetch<G>(p, L, V, l)) break;
         if (kuse(p.force_ssa)) {
-            if (L.t >= p.tmax || gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);
+            if (L.t >= p.tmax || gssa<G, PF>(p, L, V, l)) gfinish<G>(p, L, V, l);
             continue;
         }
         if (!pend) {
             CTL_PHASE(PH_PRE_LEAP);
-            const int r = gleap<G>(p, L, V, l, recip);
+            const int r = gleap<G, PF>(p, L, V, l, recip);
             if (r == LEAP_DONE || r == LEAP_ABSORB) {
                 gfinish<G>(p, L, V, l);
             } else if (r == LEAP_SWITCH) {
@@ -1202,7 +1286,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G
         if (go && pend) {
             pend = false;
             CTL_PHASE(PH_PRE_SSA);
-            if (gssa<G>(p, L, V, l)) gfinish<G>(p, L, V, l);
+            if (gssa<G, PF>(p, L, V, l)) gfinish<G>(p, L, V, l);
         }
     }
     // before parking: the burst of a switched simulation runs to its end (a
@@ -1210,9 +1294,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G
     // force-SSA mode)
     if (rg) {
         if (L.sim >= 0 && (L.ssa_left > 0 || (L.ssa_mode && !kuse(p.force_ssa))) &&
-            gssa_run<G>(p, L, V, l, L.ssa_left > 0 ? L.ssa_left : p.ssa_steps))
+            gssa_run<G, PF>(p, L, V, l, L.ssa_left > 0 ? L.ssa_left : p.ssa_steps))
             gfinish<G>(p, L, V, l);
-    } else if (pend && gssa<G>(p, L, V, l)) {
+    } else if (pend && gssa<G, PF>(p, L, V, l)) {
         gfinish<G>(p, L, V, l);
     }
     const int s0 = L.slot;
@@ -1267,17 +1351,20 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
         oe = (int)T.size();
         T.insert(T.end(), e.begin(), e.end());
     };
-    // the factor table lives in the Kc region, free outside the count and
-    // update phases; a model whose table would not fit there (2S + 1 > Rp)
-    // computes propensities from the state instead of enlarging every
-    // group's LDS (which would cost occupancy: the enzyme cascade, 61 species
-    // and 60 reactions at G = 4, ran 20 % slower with a separate table)
-    p.pfast = 2 * m.S + 1 <= group_rpad(m.R, G) ? 1 : 0;
+    // propensity mode (pmode): with at most two reactant entries of
+    // multiplicity <= 2 per reaction, two table entries per reaction -- factor
+    // indices (1) when the factor table fits in the Kc region, free outside
+    // the count and update phases, else species | multiplicity read from the
+    // state (2): a separate table would enlarge every group's LDS and cost
+    // occupancy (the enzyme cascade, 61 species and 60 reactions at G = 4,
+    // ran 20 % slower with one)
+    bool two = true;
     for (int j = 0; j < m.R; ++j) {
-        if (m.Lptr[j + 1] - m.Lptr[j] > 2) p.pfast = 0;
+        if (m.Lptr[j + 1] - m.Lptr[j] > 2) two = false;
         for (uint32_t h = m.Lptr[j]; h < m.Lptr[j + 1]; ++h)
-            if ((m.Lent[h] >> 16) > 2) p.pfast = 0;
+            if ((m.Lent[h] >> 16) > 2) two = false;
     }
+    p.pfast = !two ? 0 : (2 * m.S + 1 <= group_rpad(m.R, G) ? 1 : 2);
     std::vector<int> rorder((size_t)((m.R + G - 1) / G) * G, -1);
     for (int j = 0; j < m.R; ++j) rorder[j] = j;          // reaction j: lane j mod G (the selection's partials)
     // pfast: a reactant entry becomes its index in the factor table F
@@ -1285,22 +1372,23 @@ inline void append_group_streams(std::vector<uint32_t>& T, KParams& p, int G, co
     const uint32_t one = (uint32_t)(2 * m.S);
     auto findex = [&](uint32_t e) { const uint32_t v = e >> 16, sp = e & 0xffffu;
                                     return v == 1u ? sp : (v == 2u ? (uint32_t)m.S + sp : one); };
-    if (p.pfast) {
+    if (p.pfast == 1) {
         std::vector<uint32_t> fent(m.Lent.size());
         for (size_t h = 0; h < m.Lent.size(); ++h) fent[h] = findex(m.Lent[h]);
         rounds(rorder, m.Lptr, fent, p.oPRm, p.oPRe, 1, 2, one);
     } else {
-        rounds(rorder, m.Lptr, m.Lent, p.oPRm, p.oPRe, 1, 0, 0u);
+        rounds(rorder, m.Lptr, m.Lent, p.oPRm, p.oPRe, 1, p.pfast == 2 ? 2 : 0, 0u);
     }
     p.oDR = 0;
     if (p.pfast) {                        // dependent records, 16-byte aligned
         while (T.size() % 4) T.push_back(0u);
         p.oDR = (int)T.size();
-        for (uint32_t r : Dent) {
+        for (uint32_t r : Dent) {       // factor indices (mode 1) or the entries themselves (mode 2)
             const uint32_t n = m.Lptr[r + 1] - m.Lptr[r];
+            const uint32_t e0 = n > 0 ? m.Lent[m.Lptr[r]] : 0u, e1 = n > 1 ? m.Lent[m.Lptr[r] + 1] : 0u;
             T.push_back(r);
-            T.push_back(n > 0 ? findex(m.Lent[m.Lptr[r]]) : one);
-            T.push_back(n > 1 ? findex(m.Lent[m.Lptr[r] + 1]) : one);
+            T.push_back(p.pfast == 1 ? (n > 0 ? findex(e0) : one) : e0);
+            T.push_back(p.pfast == 1 ? (n > 1 ? findex(e1) : one) : e1);
             T.push_back(0u);
         }
     }

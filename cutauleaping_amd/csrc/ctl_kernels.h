@@ -81,9 +81,11 @@ struct KParams {
     // first.  gsimple: no species has HOR 3 or HOR 2 of type 2, so g is 1 or 2.
     int oPRm, oPRe, oVRm, oVRe, oFRm, oFRe, oSPI, gsimple;
     int vhalf;                // var entries as FP16 coefficient << 16 | byte offset of A[reaction] (every |v| <= 2048)
-    int oDR;                  // pfast: SSA dependent records {r, factor index 0, factor index 1, 0} (uint4, Dptr order)
-    int pfast;                // every reaction has <= 2 reactant entries of multiplicity <= 2:
-                              //   the reactant stream then has exactly 2 entries per round
+    int oDR;                  // pfast 1 / 2: SSA dependent records {r, entry 0, entry 1, 0} (uint4, Dptr order),
+                              //   the entries as factor indices (1) or species | multiplicity << 16 (2)
+    int pfast;                // propensity mode: 0 from the reactant entries; with <= 2 reactant entries of
+                              //   multiplicity <= 2 per reaction (exactly 2 entries per round): 1 factor
+                              //   indices into the LDS factor table, 2 entries read from the state
     const uint32_t* horw;     // hor | hort << 8
     const int32_t* cs;
     const double* ts;

@@ -140,14 +140,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
     store_lane(p, L);
 }
 
-const void* group_kernel_fn(int G, bool rg) {
+// The group kernel instance for G lanes per simulation, with or without
+// regrouping, for propensity mode pf (KParams::pfast: 0, 1 or 2).
+template <int G, bool RG>
+const void* group_kernel_fn_p(int pf) {
+    switch (pf) {
+        case 1: return (const void*)group_kernel<G, RG, 1>;
+        case 2: return (const void*)group_kernel<G, RG, 2>;
+    }
+    return (const void*)group_kernel<G, RG, 0>;
+}
+
+template <int G>
+const void* group_kernel_fn_g(bool rg, int pf) {
+    if constexpr (G < 64) {
+        if (rg) return group_kernel_fn_p<G, true>(pf);
+    }
+    return group_kernel_fn_p<G, false>(pf);
+}
+
+const void* group_kernel_fn(int G, bool rg, int pf) {
     switch (G) {
-        case 2: return rg ? (const void*)group_kernel<2, true> : (const void*)group_kernel<2, false>;
-        case 4: return rg ? (const void*)group_kernel<4, true> : (const void*)group_kernel<4, false>;
-        case 8: return rg ? (const void*)group_kernel<8, true> : (const void*)group_kernel<8, false>;
-        case 16: return rg ? (const void*)group_kernel<16, true> : (const void*)group_kernel<16, false>;
-        case 32: return rg ? (const void*)group_kernel<32, true> : (const void*)group_kernel<32, false>;
-        case 64: return (const void*)group_kernel<64, false>;
+        case 2: return group_kernel_fn_g<2>(rg, pf);
+        case 4: return group_kernel_fn_g<4>(rg, pf);
+        case 8: return group_kernel_fn_g<8>(rg, pf);
+        case 16: return group_kernel_fn_g<16>(rg, pf);
+        case 32: return group_kernel_fn_g<32>(rg, pf);
+        case 64: return group_kernel_fn_g<64>(false, pf);
     }
     return nullptr;
 }
@@ -524,7 +543,7 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
         s->smem = (size_t)gpb * gwords * 4;
         s->lds = true;
         s->compact = compact > 0 && G < 64 && s->block > 64 ? compact : 0;
-        const void* kfn = group_kernel_fn(G, s->compact > 0);
+        const void* kfn = group_kernel_fn(G, s->compact > 0, 1);   // occupancy (the instances alike)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, s->block, s->smem);
         if (e != hipSuccess || per_cu <= 0) per_cu = 1;
         long long groups = o->slots > 0 ? (long long)o->slots : (long long)prop.multiProcessorCount * per_cu * gpb;
@@ -739,7 +758,7 @@ int ctl_sim_run(ctl_sim* s) {
                     hipLaunchKernelGGL(tau_leap_kernel<false>, dim3(s->grid), dim3(s->block), s->smem, s->stream, p);
                 break;
             default: {
-                const void* fn = group_kernel_fn(s->lanes, s->compact > 0);
+                const void* fn = group_kernel_fn(s->lanes, s->compact > 0, p.pfast);
                 if (!fn) return ctl::fail(CTL_E_STATE, "bad lanes_per_sim");
                 void* args[] = {&p};
                 HIPCHK(hipLaunchKernel(fn, dim3(s->grid), dim3(s->block), args, s->smem, s->stream),

@@ -163,3 +163,19 @@ def test_feed_loader(ctl, tmp_path, capfd):
     with pytest.raises(ctl.CtlError):
         m.set_feeds([5], [[1]])
     m.set_feeds([], np.zeros((1, 0), np.uint32))
+
+
+def test_phase_clock_only_in_the_profiling_build(ctl):
+    """ctl_phase_clock is exported by both builds; the normal library
+    (libctl.so) has no phase clock and says so without touching a device,
+    and the profiling build exports the same symbols."""
+    import ctypes as C
+    buf = np.zeros(16, np.uint64)
+    assert ctl.lib().ctl_phase_clock(0, buf.ctypes.data, 16, 0) == ctl.CTL_E_STATE
+    assert b"phase clock" in ctl.lib().ctl_last_error()
+    prof = os.path.join(os.path.dirname(ctl.LIB_PATH), "libctl_phases.so")
+    if not os.path.exists(prof):
+        pytest.skip("profiling build not built")
+    P = C.CDLL(prof)
+    for name in ctl.EXPORTS:
+        assert hasattr(P, name), name
