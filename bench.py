@@ -360,7 +360,9 @@ def bench_c0(args):
                                              "slots", "block_size", "lds_state", "lanes_per_sim")},
              "roofline": model_rooflines(model, st, kms / steps, n)}
     if not args.no_cpu:
-        extra["cpu_baseline"] = cpu_baseline(net, ts, min(n, 4096), locked=[1, 0, 1])
+        # about 10 s of host work: the rate per firing does not depend on how
+        # many simulations run, so the sample is 256 times the GPU's 1024
+        extra["cpu_baseline"] = cpu_baseline(net, ts, 256 * n, locked=[1, 0, 1])
     _line("reaction firings/sec on 1 GPU (Schlogl)", fir / dt, "firings/s", steps, dt * 1e3 / steps, "f32",
           {"workload": f"Schlogl (X with buffered A=1e5, B=2e5), {n} simulations, 100 samples to t_end=10",
            "simulations": n, "t_end": 10.0, "compact_interval": args.compact}, **extra)
@@ -525,8 +527,8 @@ def bench_c4(args):
                 st = s.stats()
                 fir += st["firings"]
         extra = {}
-        if not args.no_cpu:
-            extra["cpu_baseline"] = cpu_baseline(net, ts, min(n, args.cpu_sims), force_ssa=force)
+        if not args.no_cpu:     # about 10 s of host work per mode (the exact SSA costs the host 13x more)
+            extra["cpu_baseline"] = cpu_baseline(net, ts, args.cpu_sims * (4 if force else 64), force_ssa=force)
         _line("reaction firings/sec on 1 GPU (enzyme cascade)", fir / (ms / 1e3), "firings/s", args.steps,
               ms / args.steps, "f32",
               {"workload": f"enzyme cascade 20 stages (61 species, 60 reactions), {n} simulations, 100 samples "
