@@ -694,7 +694,7 @@ int ctl_sim_create(const ctl_model* mh, const ctl_opts* o, ctl_sim** out) {
     p.ssa_steps = o->ssa_steps;
     p.ssa_factor = o->ssa_factor;
     p.force_ssa = o->force_ssa ? 1 : 0;
-    p.ssa_chunk = o->ssa_chunk > 0 ? o->ssa_chunk : (G > 1 ? 6 : 10);
+    p.ssa_chunk = o->ssa_chunk > 0 ? o->ssa_chunk : (G > 1 ? 8 : 10);
     p.compact = s->compact;
     p.iters = o->iters_per_launch > 0 ? o->iters_per_launch : (G > 1 ? 65536 : 4096);
     p.ns = ns;

@@ -76,12 +76,13 @@ typedef struct {
     uint64_t seed;             /* Philox key (the reference used time(NULL), K.cu:176) */
     int32_t  verbose;
     /* launch geometry (0 = chosen by the library) */
-    int32_t  block_size;       /* threads per workgroup (64..256, multiple of 64; 0: library default,
-                                  256 where LDS allows) */
+    int32_t  block_size;       /* threads per workgroup (64..256, multiple of 64; 0: library default;
+                                  < 0 with one lane per simulation: default size, state kept in HBM
+                                  even when it would fit in LDS) */
     int32_t  slots;            /* resident simulation slots */
     int32_t  iters_per_launch; /* leap/SSA iterations per slot per launch */
     int32_t  ssa_chunk;        /* SSA steps per iteration of a slot (0: 10 for one lane per
-                                  simulation, 6 for a regrouping group kernel) */
+                                  simulation, 8 for a regrouping group kernel) */
     int32_t  lanes_per_sim;    /* 1: one lane runs one simulation; 2..64 (power of two): that many
                                   lanes of a wavefront cooperate on one simulation whose state
                                   stays in LDS (the group algorithm); 0: chosen by the library */

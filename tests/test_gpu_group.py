@@ -201,8 +201,9 @@ def test_group_batch_uneven_models_bit_exact(gpu, oracle_lib, force):
 def test_autotune_picks_a_measured_candidate(gpu, tmp_path):
     """The autotuner times every lanes-per-simulation candidate on probe
     windows spread over the horizon, refines the winner's regrouping
-    interval, SSA chunk and iteration budget, and returns the fastest
-    configuration measured; the result is memoised on disk."""
+    interval, SSA chunk and iteration budget, confirms regrouping against
+    no regrouping on longer windows, and returns the fastest configuration
+    measured; the result is memoised on disk."""
     from cutauleaping_amd import autotune, networks
     print("case: autotune", flush=True)
     net = networks.mass_action(100, 200, seed=1)
@@ -214,7 +215,11 @@ def test_autotune_picks_a_measured_candidate(gpu, tmp_path):
     stage1 = [res["scores"][f"lanes_per_sim={g}"] for g in (1, 8, 16, 64)]
     assert min(stage1) > 0
     assert len(res["scores"]) > 4                          # stage 2 ran
-    assert res["scores"][autotune._key(res["options"])] == max(res["scores"].values())
+    if res["confirm"]:                                     # stage 4: the longer windows decide
+        assert len(res["confirm"]) == 2
+        assert res["confirm"][autotune._key(res["options"])] == max(res["confirm"].values())
+    else:
+        assert res["scores"][autotune._key(res["options"])] == max(res["scores"].values())
     assert res["probe"]["windows"] == 3 and res["probe"]["window_starts"][-1] > 30.0
     autotune._memo.clear()
     again = autotune.tune(net["left"], net["right"], net["x0"], net["c"], ts, probe_sims=8192, probe_fraction=0.01,
