@@ -103,7 +103,7 @@ def main():
                          "c4: enzyme cascade (adaptive and force-SSA); mlp: the surrogate network's kernels alone")
     ap.add_argument("--mlp-rows", type=int, default=65536, help="mlp: rows (parameter sets) per pass")
     ap.add_argument("--models", type=int, default=8192, help="c2: models in the batch")
-    ap.add_argument("--no-tune", action="store_true", help="c2: the library's size rule instead of the autotuner")
+    ap.add_argument("--no-tune", action="store_true", help="c0, c2, c4: the library's defaults instead of the autotuner")
     ap.add_argument("--c3-train", type=int, default=65536)
     ap.add_argument("--c3-test", type=int, default=4096)
     ap.add_argument("--c3-replicas", type=int, default=256)
@@ -274,13 +274,22 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
 
 
+_AFFINITY = os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else None
+
+
+def _restore_affinity():
+    """OpenMP with OMP_PROC_BIND binds the calling thread to its first place
+    (one core) for the rest of the process: give it back the CPUs it started
+    with, so that later GPU runs and baselines see the whole share."""
+    if _AFFINITY is not None:
+        os.sched_setaffinity(0, _AFFINITY)
+
+
 def host_cores():
-    """Threads for the CPU baseline: the CPUs this process may run on, at most
-    16 (a one-GPU share of the host), with nproc and the lscpu model."""
-    try:
-        avail = len(os.sched_getaffinity(0))
-    except AttributeError:
-        avail = os.cpu_count() or 1
+    """Threads for the CPU baseline: the CPUs this process may run on (as
+    at start-up), at most 16 (a one-GPU share of the host), with nproc and
+    the lscpu model."""
+    avail = len(_AFFINITY) if _AFFINITY is not None else (os.cpu_count() or 1)
     model, sockets, cps, tpc = "unknown", None, None, None
     try:
         import subprocess
@@ -311,8 +320,11 @@ def cpu_baseline(net, ts, nsims, locked=None, cs=None, force_ssa=False):
     import oracle
     threads, host = host_cores()
     t0 = time.perf_counter()
-    _, st = oracle.run(net["left"], net["right"], net["x0"], net["c"], ts, n=nsims, seed=0xC0FFEE,
-                       locked=locked, cs=cs, force_ssa=force_ssa, precision=oracle.REFSEM, threads=threads)
+    try:
+        _, st = oracle.run(net["left"], net["right"], net["x0"], net["c"], ts, n=nsims, seed=0xC0FFEE,
+                           locked=locked, cs=cs, force_ssa=force_ssa, precision=oracle.REFSEM, threads=threads)
+    finally:
+        _restore_affinity()
     dt = time.perf_counter() - t0
     return {"value": float(st["firings"].sum()) / dt, "unit": "firings/s", "cores": threads,
             "kind": "port", "sample": f"{nsims} simulations of the same workload (full t_end), "
@@ -331,10 +343,10 @@ def _line(metric, value, unit, steps, ms, dtype, config, **extra):
 def bench_c0(args):
     """Schlogl (the reference repository's CPU-runnable configuration): the
     four-reaction model of tests/golden/schlogl (A and B buffered), 1024
-    simulations (--c0-sims) to t_end=10 with 100 samples, one lane per
-    simulation with its state in LDS, leap-mode and SSA-mode simulations
-    regrouped into separate wavefronts (--compact); a step is one complete run
-    of the simulations."""
+    simulations (--c0-sims) to t_end=10 with 100 samples, with the
+    autotuner's launch shape (untimed; --no-tune, --lanes or --compact for
+    the library's: one lane per simulation with its state in LDS); a step is
+    one complete run of the simulations."""
     import cutauleaping_amd as ctl
     left = [[1, 2, 0], [0, 3, 0], [0, 0, 1], [0, 1, 0]]
     right = [[0, 3, 0], [1, 2, 0], [0, 1, 0], [0, 0, 1]]
@@ -344,7 +356,15 @@ def bench_c0(args):
     n = max(64, args.c0_sims // 64 * 64)
     steps = max(args.steps, 1)
     fir, kms, launches = 0, 0.0, 0
-    with ctl.Sim(model, 64, n // 64, seed=0xC0FFEE, lanes_per_sim=args.lanes, compact_interval=args.compact) as s:
+    opts, tuned = {"lanes_per_sim": args.lanes, "compact_interval": args.compact}, None
+    if args.lanes == 0 and args.compact == 0 and not args.no_tune:    # untimed: the autotuner's launch shape
+        from cutauleaping_amd import autotune
+        t0 = time.time()
+        r = autotune.tune(left, right, net["x0"], net["c"], ts, feed=[1, 0, 1], probe_sims=n, window_sims=n)
+        opts = dict(r["options"])
+        tuned = {"options": opts, "seconds": round(time.time() - t0, 2), "confirm": r["confirm"]}
+        print(f"# autotuned in {tuned['seconds']} s: {opts}", file=sys.stderr, flush=True)
+    with ctl.Sim(model, 64, n // 64, seed=0xC0FFEE, **opts) as s:
         for _ in range(args.warmup):
             s.run()
         t1 = time.perf_counter()
@@ -359,13 +379,15 @@ def bench_c0(args):
              "counters": {k: st[k] for k in ("leaps", "ssa_steps", "switches", "poisson", "gaps", "launches",
                                              "slots", "block_size", "lds_state", "lanes_per_sim")},
              "roofline": model_rooflines(model, st, kms / steps, n)}
+    if tuned:
+        extra["autotune"] = tuned
     if not args.no_cpu:
         # about 10 s of host work: the rate per firing does not depend on how
         # many simulations run, so the sample is 256 times the GPU's 1024
         extra["cpu_baseline"] = cpu_baseline(net, ts, 256 * n, locked=[1, 0, 1])
     _line("reaction firings/sec on 1 GPU (Schlogl)", fir / dt, "firings/s", steps, dt * 1e3 / steps, "f32",
           {"workload": f"Schlogl (X with buffered A=1e5, B=2e5), {n} simulations, 100 samples to t_end=10",
-           "simulations": n, "t_end": 10.0, "compact_interval": args.compact}, **extra)
+           "simulations": n, "t_end": 10.0, "options": opts}, **extra)
     model.close()
 
 
@@ -380,7 +402,7 @@ def bench_c2(args):
     from cutauleaping_amd import networks
     t_end = args.t_end if args.t_end != 50.0 else 0.1
     t0 = time.time()
-    procs = max(1, min(16, len(os.sched_getaffinity(0))))
+    procs = max(1, min(16, host_cores()[0]))
     nets = networks.batched_models(args.models, seed=2, t_end=t_end, processes=procs,
                                    log=lambda m: print(m, file=sys.stderr, flush=True))
     print(f"# {len(nets)} models generated in {time.time() - t0:.1f}s on {procs} processes", file=sys.stderr,
@@ -452,8 +474,11 @@ def c2_cpu_baseline(nets, ts, every=512, per_model=16):
     for n in nets[::every]:
         k = min(int(n["replicas"]), per_model)
         t0 = time.perf_counter()
-        _, st = oracle.run(n["left"], n["right"], n["x0"], n["c"], ts, cs=n["cs"], n=k, seed=0xC0FFEE,
-                           precision=oracle.REFSEM, threads=threads)
+        try:
+            _, st = oracle.run(n["left"], n["right"], n["x0"], n["c"], ts, cs=n["cs"], n=k, seed=0xC0FFEE,
+                               precision=oracle.REFSEM, threads=threads)
+        finally:
+            _restore_affinity()
         secs += time.perf_counter() - t0
         fir += int(st["firings"].sum())
         nsim += k
@@ -506,7 +531,9 @@ def bench_c1ref(args):
 def bench_c4(args):
     """Enzyme cascade: networks.enzyme_cascade(20, seed=4) (61 species, 60
     reactions), 2^18 simulations, 100 samples to t_end=100, adaptive and
-    force-SSA (two lines); at least one untimed run before the timed ones."""
+    force-SSA (two lines), each with the autotuner's launch shape (untimed;
+    --no-tune or --lanes for the library's size rule); at least one untimed
+    run before the timed ones."""
     import cutauleaping_amd as ctl
     from cutauleaping_amd import networks
     net = networks.enzyme_cascade(20, seed=4)
@@ -516,8 +543,16 @@ def bench_c4(args):
     n = min(args.sims, 1 << 18)
     for force in (False, True):
         fir, ms = 0, 0.0
-        with ctl.Sim(model, 256, n // 256, seed=0xC0FFEE, force_ssa=force, lanes_per_sim=args.lanes,
-                     verbose=True) as s:
+        opts, tuned = {"lanes_per_sim": args.lanes}, None
+        if args.lanes == 0 and not args.no_tune:     # untimed: the autotuner's launch shape for this mode
+            from cutauleaping_amd import autotune
+            t0 = time.time()
+            r = autotune.tune(net["left"], net["right"], net["x0"], net["c"], ts, probe_sims=min(n, 65536),
+                              force_ssa=force)
+            opts = dict(r["options"])
+            tuned = {"options": opts, "seconds": round(time.time() - t0, 2), "confirm": r["confirm"]}
+            print(f"# autotuned force_ssa={int(force)} in {tuned['seconds']} s: {opts}", file=sys.stderr, flush=True)
+        with ctl.Sim(model, 256, n // 256, seed=0xC0FFEE, force_ssa=force, verbose=True, **opts) as s:
             for k in range(max(args.warmup, 1)):      # untimed: the first run also loads the code object
                 s.run()
             for k in range(args.steps):
@@ -526,9 +561,9 @@ def bench_c4(args):
                 ms += (time.perf_counter() - t1) * 1e3
                 st = s.stats()
                 fir += st["firings"]
-        extra = {}
-        if not args.no_cpu:     # about 10 s of host work per mode (the exact SSA costs the host 13x more)
-            extra["cpu_baseline"] = cpu_baseline(net, ts, args.cpu_sims * (4 if force else 64), force_ssa=force)
+        extra = {"autotune": tuned} if tuned else {}
+        if not args.no_cpu:     # about 10 s of host work per mode (both cost the host about the same per firing)
+            extra["cpu_baseline"] = cpu_baseline(net, ts, args.cpu_sims * 64, force_ssa=force)
         _line("reaction firings/sec on 1 GPU (enzyme cascade)", fir / (ms / 1e3), "firings/s", args.steps,
               ms / args.steps, "f32",
               {"workload": f"enzyme cascade 20 stages (61 species, 60 reactions), {n} simulations, 100 samples "

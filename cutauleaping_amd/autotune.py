@@ -5,13 +5,26 @@ interval, SSA chunk, workgroup size, iterations per launch) are rules of
 thumb.  tune() instead times short probe runs of the actual model on the
 actual device and keeps the fastest configuration:
 
-1. Windows across the horizon.  One run of the library's default over
-   `window_sims` simulations records every species at the window start
-   times w * T / windows (w = 1 .. windows-1).  Probe window w restarts the
-   model from those states (window 0 from the initial state) and runs
-   `probe_fraction` of the horizon: a mass-action network is
-   time-homogeneous, so a window is the model from that point on.  Early
-   transients and late quasi-stationary phases (often SSA-heavy) both count.
+1. The whole horizon, cut into windows and mixed in one launch.  One run of
+   the library's default over `window_sims` simulations records every
+   species at w * T / windows (w = 1 .. windows-1).  A probe of W windows
+   (W divides `windows`) restarts the model from the states at w * T / W
+   (and from the initial state) and runs each for T / W: a mass-action
+   network is time-homogeneous, so a restarted simulation is the model from
+   that point on, and the W windows together cover every regime of the run
+   (start-up, transitions, the quasi-stationary phase) in proportion.  The
+   windows are interleaved simulation by simulation in ONE launch, because a
+   whole run's workgroups hold simulations at different points of their
+   trajectories (a group that finishes a simulation starts its next one from
+   the initial state while its neighbours are deep in theirs): the mix of
+   leaping and SSA simulations inside a workgroup is what regrouping by mode
+   separates.  W starts at `windows` and is cut until the library default's
+   probe launch takes `min_probe_ms` (a short launch times mostly its start
+   and its end).  The windows' costs differ (an SSA-heavy window can cost ten
+   times a leaping one), and a launch whose last simulations are the
+   costliest times mostly idle lanes: the calibration run's per-simulation
+   counters rank the windows, and the probe hands out the costliest quarter
+   first and the cheapest last.
 2. Stage 1 times every lanes-per-simulation candidate (1, and G = 2..64 where
    the model fits the group kernel's LDS) with the library's defaults.
 3. Stage 2 refines the winner: for G = 2..32 the regrouping interval
@@ -19,14 +32,16 @@ actual device and keeps the fastest configuration:
    two wavefronts instead of four; for G = 1 the workgroup size with and
    without regrouping, then the SSA chunk; for every G the iteration budget
    per launch.
-4. For G = 2..32, the best configuration with regrouping and the best
-   without are timed again on the first two windows made `confirm` times
-   longer: regrouping pays once simulations have drifted apart in mode,
-   which short windows under-represent (on the 100x200 network the short
-   windows favour no regrouping, whole runs favour it by 10 %).
+4. Stage 4 times the rivals on fewer, longer windows -- the whole horizon
+   when the estimate fits `confirm_ms` -- and keeps the fastest: the best
+   configuration, the library's defaults at its G, the best one of its G on
+   the other side of regrouping, and the runner-up G of stage 1 when it came
+   within 10 %.  Short windows cut
+   simulations off their neighbours' regimes, and the decision that depends
+   most on that mixing (regrouping) is the one confirmed.
 
-A configuration's score is its total firings over its total kernel time
-across the windows (firings per second of the whole probe).  Results are
+A configuration's score is its firings over its kernel time (firings per
+second of the probe launch).  Results are
 memoised per (model signature, device, probe settings) in-process and
 optionally in a JSON file.  Lanes per simulation selects one of several
 statistically equivalent algorithms, so the trajectories of a seed depend on
@@ -72,9 +87,10 @@ def _key(opts: dict) -> str:
     return ",".join(f"{k}={v}" for k, v in sorted(opts.items()))
 
 
-def _window_starts(left, right, x0, c, ts, feed, windows, window_sims, seed, gpu):
-    """Initial-state rows of each probe window: x0 for window 0, then the
-    states of `window_sims` default-configuration simulations at w*T/windows."""
+def _grid_states(left, right, x0, c, ts, feed, windows, window_sims, seed, gpu, force_ssa=False):
+    """States at the start of each of `windows` equal windows of the horizon:
+    [x0 (1 row)] + [window_sims rows at w*T/windows, w = 1 .. windows-1], from
+    one run of the library's default configuration."""
     starts = [np.atleast_2d(np.asarray(x0))[:1].astype(np.uint32)]
     if windows <= 1:
         return starts
@@ -82,7 +98,7 @@ def _window_starts(left, right, x0, c, ts, feed, windows, window_sims, seed, gpu
     t_w = np.array([T * w / windows for w in range(1, windows)])
     m = Model.from_arrays(left, right, np.atleast_2d(x0)[:1], np.atleast_2d(c)[:1], t_w, feed=feed)
     try:
-        with Sim(m, 64, max(1, window_sims // 64), gpu=gpu, seed=seed ^ 0x5A5A) as s:
+        with Sim(m, 64, max(1, window_sims // 64), gpu=gpu, seed=seed ^ 0x5A5A, force_ssa=force_ssa) as s:
             s.run()
             X = s.samples()                          # [sims][windows-1][S], every species
     finally:
@@ -91,22 +107,53 @@ def _window_starts(left, right, x0, c, ts, feed, windows, window_sims, seed, gpu
     return starts
 
 
-def tune(left, right, x0, c, ts, *, feed=None, cs=None, gpu=0, probe_sims=65536, probe_fraction=0.02,
-         windows=4, window_sims=1024, candidates=CANDIDATES, refine=True, confirm=5, seed=0x7E57,
+def _mixed(starts, n, order=None):
+    """n initial-state rows from the window starts.  Without `order`, row i
+    takes start i % W (within a start, its rows in turn).  With `order` (the
+    windows by decreasing cost), the rows come in four consecutive quarters,
+    each interleaving a quarter of the windows, the costliest first: the
+    persistent kernels fetch simulations in index order, so the launch ends
+    on the cheapest windows instead of waiting on a few of the costliest."""
+    W = len(starts)
+    i = np.arange(n)
+    w_of, k_of = i % W, i // W
+    if order is not None:
+        rank = np.empty(W, np.int64)
+        rank[np.asarray(order)] = np.arange(W)
+        r = rank[w_of]
+        idx = np.lexsort((r, k_of, r // max(1, W // 4)))   # quarter, then round-robin inside it
+        w_of, k_of = w_of[idx], k_of[idx]
+    X = np.empty((n, starts[0].shape[1]), np.uint32)
+    for w, st in enumerate(starts):
+        sel = w_of == w
+        X[sel] = st[k_of[sel] % st.shape[0]]
+    return X
+
+
+def _divisor_at_most(n: int, x: float) -> int:
+    """Largest divisor of n that is <= x (at least 1)."""
+    return max([d for d in range(1, n + 1) if n % d == 0 and d <= x] or [1])
+
+
+def tune(left, right, x0, c, ts, *, feed=None, cs=None, gpu=0, probe_sims=65536, windows=64, min_probe_ms=250.0,
+         window_sims=1024, candidates=CANDIDATES, refine=True, confirm_ms=15000.0, force_ssa=False, seed=0x7E57,
          cache_file=None, verbose=False) -> dict:
     """Returns {"lanes_per_sim": G, "options": Sim keyword arguments of the
-    fastest configuration, "scores": {configuration: firings/s on the probe
-    windows}, "confirm": {configuration: firings/s on the longer windows}
-    (empty when stage 4 did not run), "probe": {...}}."""
+    fastest configuration, "scores": {configuration: firings/s on the
+    probe}, "confirm": {configuration: firings/s on the confirmation run}
+    (empty when stage 4 did not run), "probe": {...}}.  force_ssa tunes the
+    exact-SSA mode (every probe, the window states included, runs in it)."""
     left = np.asarray(left)
     R, S = left.shape
     ts = np.asarray(ts, np.float64)
+    T = float(ts[-1])
     sig = signature(left, right, x0, c, ts, feed)
-    key = (sig, gpu, probe_sims, probe_fraction, windows, tuple(candidates), refine, confirm)
+    force_ssa = bool(force_ssa)
+    key = (sig, gpu, probe_sims, windows, min_probe_ms, tuple(candidates), refine, confirm_ms, force_ssa)
     if key in _memo:
         return _memo[key]
-    disk_key = (f"{sig}@{gpu}:{probe_sims}:{probe_fraction}:{windows}:{','.join(map(str, candidates))}:"
-                f"{int(refine)}:{confirm}")
+    disk_key = (f"{sig}@{gpu}:{probe_sims}:{windows}:{min_probe_ms:g}:{','.join(map(str, candidates))}:"
+                f"{int(refine)}:{confirm_ms:g}:{int(force_ssa)}")
     if cache_file and os.path.exists(cache_file):
         with open(cache_file) as f:
             hit = json.load(f).get(disk_key)
@@ -114,17 +161,49 @@ def tune(left, right, x0, c, ts, *, feed=None, cs=None, gpu=0, probe_sims=65536,
             _memo[key] = hit
             return hit
     say = (lambda m: print(m, flush=True)) if verbose else (lambda m: None)
+    windows = max(1, int(windows))
     window_sims = max(64, window_sims // 64 * 64)
-    probe_sims = max(window_sims, probe_sims // window_sims * window_sims)    # whole replica groups per row
+    probe_sims = max(64, probe_sims // 64 * 64)
     t0 = time.perf_counter()
-    starts = _window_starts(left, right, x0, c, ts, feed, windows, window_sims, seed, gpu)
-    dt = float(ts[-1]) * probe_fraction
-    t_probe = ts[ts <= dt + 1e-12]
-    if len(t_probe) == 0:
-        t_probe = np.array([dt])
-    models = [Model.from_arrays(left, right, st, np.atleast_2d(c)[:1], t_probe, feed=feed, cs=cs) for st in starts]
-    say(f"# autotune: {len(models)} windows of {dt:g} time units, {probe_sims} simulations each "
-        f"({time.perf_counter() - t0:.1f} s to set up)")
+    states = _grid_states(left, right, x0, c, ts, feed, windows, window_sims, seed, gpu, force_ssa)
+
+    def probe_model(W, order=None):
+        """W windows of T/W each, starting at w*T/W, mixed in one model."""
+        dt = T / W
+        t_p = ts[ts <= dt * (1 + 1e-9)]
+        if len(t_p) == 0:
+            t_p = np.array([dt])
+        X = _mixed(states[::windows // W], probe_sims, order)
+        return Model.from_arrays(left, right, X, np.atleast_2d(c)[:1], t_p, feed=feed, cs=cs)
+
+    def timed(m, opts, per_sim=False):
+        with Sim(m, 64, probe_sims // 64, gpu=gpu, seed=seed, force_ssa=force_ssa, **opts) as s:
+            s.run()
+            st = s.stats()
+            ss = s.sim_stats() if per_sim else None
+        return st["firings"], st["kernel_ms"], ss
+
+    # Calibration: fewer, longer windows until the library default's probe
+    # launch takes min_probe_ms (short launches time mostly their start and end).
+    W, calib = windows, []
+    while True:
+        m = probe_model(W)
+        try:
+            fir, ms, ss = timed(m, {}, per_sim=True)
+        finally:
+            m.close()
+        calib.append({"windows": W, "kernel_ms": round(ms, 2)})
+        if ms >= min_probe_ms or W == 1:
+            break
+        W = _divisor_at_most(windows, min(W - 1, W * ms / min_probe_ms))
+    # Cost of a window's simulations in the calibration run (an SSA step
+    # costs about a sixth of a leap), for the costliest-first order.
+    cost = (ss["leaps"].astype(np.float64) + ss["ssa_steps"] / 6.0)
+    w_cost = np.bincount(np.arange(probe_sims) % W, weights=cost, minlength=W)
+    order = np.argsort(-w_cost, kind="stable")
+    models = [probe_model(W, order)]
+    say(f"# autotune: {W} windows of {T / W:g} time units over {probe_sims} simulations, costliest first "
+        f"(calibration {calib}; {time.perf_counter() - t0:.1f} s to set up)")
     scores = {}
     confirmed = {}
 
@@ -132,38 +211,34 @@ def tune(left, right, x0, c, ts, *, feed=None, cs=None, gpu=0, probe_sims=65536,
         k = _key(opts)
         if k in scores:
             return scores[k]
-        fir, ms = 0, 0.0
         t1 = time.perf_counter()
-        for i, m in enumerate(models):
-            try:
-                with Sim(m, 64, probe_sims // 64, gpu=gpu, seed=seed + i, **opts) as s:
-                    s.run()
-                    st = s.stats()
-            except CtlError as e:        # a shape the model does not allow (LDS)
-                say(f"# autotune {k}: not possible ({e})")
-                scores[k] = 0.0
-                return 0.0
-            fir += st["firings"]
-            ms += st["kernel_ms"]
+        try:
+            fir, ms, _ = timed(models[0], opts)
+        except CtlError as e:            # a shape the model does not allow (LDS)
+            say(f"# autotune {k}: not possible ({e})")
+            scores[k] = 0.0
+            return 0.0
         rate = fir / max(ms, 1e-3) * 1e3
         scores[k] = rate
         say(f"# autotune {k}: {rate:.3e} firings/s ({ms:.1f} ms kernel, {time.perf_counter() - t1:.1f} s wall)")
         return rate
 
+    W_conf = None
     try:
-        best, best_rate = None, -1.0
+        best, best_rate, stage1 = None, -1.0, {}
         for G in candidates:
             if G > 1 and not _fits_group(S, R):
                 continue
             o = {"lanes_per_sim": G}
             r = score(o)
+            stage1[G] = r
             if r > best_rate:
                 best, best_rate = o, r
         if refine and best is not None:
             G = best["lanes_per_sim"]
             if 1 < G < 64:
                 grid = [{"compact_interval": -1}] + [{"compact_interval": ci, "ssa_chunk": ch}
-                                                     for ci in (4, 8, 16) for ch in (3, 6, 12)]
+                                                     for ci in (4, 8, 16) for ch in (4, 8, 12)]
             elif G == 1:
                 grid = [{"block_size": 64, "compact_interval": -1}, {"block_size": 128, "compact_interval": -1},
                         {"block_size": 256, "compact_interval": -1}, {"block_size": 128, "compact_interval": 8},
@@ -191,50 +266,48 @@ def tune(left, right, x0, c, ts, *, feed=None, cs=None, gpu=0, probe_sims=65536,
                 r = score(o)
                 if r > best_rate:
                     best, best_rate = o, r
-            if confirm > 1 and 1 < G < 64:
-                # Regrouping pays once simulations drift apart in mode, which short
-                # windows under-represent: the best configuration with and the best
-                # without it are timed again on windows `confirm` times longer.
-                def regroups(o):
-                    ci = o.get("compact_interval", 0)
-                    return ci > 0 or (ci == 0 and G >= 16 and not o.get("force_ssa"))
-                rivals = {}
+            # Stage 4: the rivals run longer windows (the whole horizon when
+            # that fits confirm_ms): the best configuration, the library's
+            # defaults at its G, the best one of its G on the other side of
+            # regrouping, and the runner-up G of stage 1 when it came within 10 %.
+            def regroups(o):
+                ci = o.get("compact_interval", 0)
+                return ci > 0 or (ci == 0 and 16 <= o["lanes_per_sim"] < 64 and not force_ssa)
+            rivals = [best] + ([{"lanes_per_sim": G}] if best != {"lanes_per_sim": G} else [])
+            if 1 < G < 64:
+                other = None
                 for k, r in scores.items():
                     o = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in k.split(",")}
-                    if o.get("lanes_per_sim") != G or r <= 0:
-                        continue
-                    side = regroups(o)
-                    if side not in rivals or r > scores[_key(rivals[side])]:
-                        rivals[side] = o
-                if len(rivals) == 2:
-                    t_long = ts[ts <= float(ts[-1]) * probe_fraction * confirm + 1e-12]
-                    if len(t_long) == 0:
-                        t_long = np.array([float(ts[-1]) * probe_fraction * confirm])
-                    long_models = [Model.from_arrays(left, right, st, np.atleast_2d(c)[:1], t_long, feed=feed, cs=cs)
-                                   for st in starts[:2]]
-                    try:
-                        long_rate = {}
-                        for side, o in rivals.items():
-                            fir, ms = 0, 0.0
-                            for i, m in enumerate(long_models):
-                                with Sim(m, 64, probe_sims // 64, gpu=gpu, seed=seed + i, **o) as s:
-                                    s.run()
-                                    st = s.stats()
-                                fir += st["firings"]
-                                ms += st["kernel_ms"]
-                            long_rate[side] = fir / max(ms, 1e-3) * 1e3
-                            confirmed[_key(o)] = long_rate[side]
-                            say(f"# autotune confirm {_key(o)}: {long_rate[side]:.3e} firings/s")
-                        best = rivals[max(long_rate, key=long_rate.get)]
-                    finally:
-                        for m in long_models:
-                            m.close()
+                    if (o.get("lanes_per_sim") == G and r > 0 and regroups(o) != regroups(best)
+                            and o not in rivals):
+                        if other is None or r > scores[_key(other)]:
+                            other = o
+                if other is not None:
+                    rivals.append(other)
+            runner = sorted((r, g) for g, r in stage1.items() if g != G and r > 0)
+            if runner and runner[-1][0] >= 0.9 * stage1[G]:
+                rivals.append({"lanes_per_sim": runner[-1][1]})
+            probe_ms = fir / max(best_rate, 1e-9) * 1e3      # the default's firings at the best rate
+            W_conf = next((d for d in range(1, W) if W % d == 0 and probe_ms * W / d <= confirm_ms), None)
+            if confirm_ms > 0 and len(rivals) > 1 and W_conf is not None:
+                # window v of W_conf spans windows v*r .. v*r+r-1 of the probe
+                c_conf = w_cost.reshape(W_conf, W // W_conf).sum(axis=1)
+                m = probe_model(W_conf, np.argsort(-c_conf, kind="stable"))
+                try:
+                    long_rate = {}
+                    for o in rivals:
+                        fir_c, ms_c, _ = timed(m, o)
+                        long_rate[_key(o)] = confirmed[_key(o)] = fir_c / max(ms_c, 1e-3) * 1e3
+                        say(f"# autotune confirm on {W_conf} window(s) {_key(o)}: {confirmed[_key(o)]:.3e} firings/s")
+                    best = max(rivals, key=lambda o: long_rate[_key(o)])
+                finally:
+                    m.close()
     finally:
         for m in models:
             m.close()
     res = {"lanes_per_sim": int(best["lanes_per_sim"]), "options": best, "scores": scores, "confirm": confirmed,
-           "probe": {"sims": probe_sims, "fraction": probe_fraction, "t_end": float(t_probe[-1]),
-                     "windows": len(starts), "window_starts": [float(ts[-1]) * w / windows for w in range(len(starts))],
+           "probe": {"sims": probe_sims, "windows": W, "t_end": T / W, "window_starts": [T * w / W for w in range(W)],
+                     "calibration": calib, "confirm_windows": W_conf if confirmed else None,
                      "seconds": time.perf_counter() - t0}}
     _memo[key] = res
     if cache_file:

@@ -199,31 +199,39 @@ def test_group_batch_uneven_models_bit_exact(gpu, oracle_lib, force):
 
 
 def test_autotune_picks_a_measured_candidate(gpu, tmp_path):
-    """The autotuner times every lanes-per-simulation candidate on probe
-    windows spread over the horizon, refines the winner's regrouping
-    interval, SSA chunk and iteration budget, confirms regrouping against
-    no regrouping on longer windows, and returns the fastest configuration
-    measured; the result is memoised on disk."""
+    """The autotuner times every lanes-per-simulation candidate on a probe
+    of windows tiling the horizon (interleaved in one launch, fewer and
+    longer windows until the launch is long enough), refines the winner's
+    regrouping interval, SSA chunk and iteration budget, confirms the
+    best configuration against its rivals (the library's defaults, the other
+    side of regrouping, a close runner-up G) on longer windows, and returns the fastest
+    configuration measured; the result is memoised on disk."""
     from cutauleaping_amd import autotune, networks
     print("case: autotune", flush=True)
     net = networks.mass_action(100, 200, seed=1)
     ts = networks.sample_times(200, 50.0)
     cache = str(tmp_path / "tune.json")
-    res = autotune.tune(net["left"], net["right"], net["x0"], net["c"], ts, probe_sims=8192, probe_fraction=0.01,
-                        windows=3, candidates=(1, 8, 16, 64), cache_file=cache, verbose=True)
+    kw = dict(probe_sims=8192, windows=16, min_probe_ms=30.0, confirm_ms=3000.0, candidates=(1, 8, 16, 64),
+              cache_file=cache)
+    res = autotune.tune(net["left"], net["right"], net["x0"], net["c"], ts, verbose=True, **kw)
     assert res["lanes_per_sim"] in (1, 8, 16, 64) and res["options"]["lanes_per_sim"] == res["lanes_per_sim"]
     stage1 = [res["scores"][f"lanes_per_sim={g}"] for g in (1, 8, 16, 64)]
     assert min(stage1) > 0
     assert len(res["scores"]) > 4                          # stage 2 ran
     if res["confirm"]:                                     # stage 4: the longer windows decide
-        assert len(res["confirm"]) == 2
+        assert 2 <= len(res["confirm"]) <= 4
         assert res["confirm"][autotune._key(res["options"])] == max(res["confirm"].values())
+        assert res["probe"]["windows"] % res["probe"]["confirm_windows"] == 0
     else:
         assert res["scores"][autotune._key(res["options"])] == max(res["scores"].values())
-    assert res["probe"]["windows"] == 3 and res["probe"]["window_starts"][-1] > 30.0
+    W = res["probe"]["windows"]
+    assert 16 % W == 0 and res["probe"]["calibration"][-1]["windows"] == W
+    assert res["probe"]["window_starts"] == [50.0 * w / W for w in range(W)]
+    cal = res["probe"]["calibration"]
+    assert all(a["kernel_ms"] < 30.0 for a in cal[:-1])   # cut only while too short
+    assert cal[-1]["kernel_ms"] >= 30.0 or W == 1
     autotune._memo.clear()
-    again = autotune.tune(net["left"], net["right"], net["x0"], net["c"], ts, probe_sims=8192, probe_fraction=0.01,
-                          windows=3, candidates=(1, 8, 16, 64), cache_file=cache)
+    again = autotune.tune(net["left"], net["right"], net["x0"], net["c"], ts, **kw)
     assert again["options"] == res["options"]
 
 
