@@ -62,6 +62,24 @@ def lane_ops(stats, S, R, nL, nV, nF, SV):
     return float(w)
 
 
+def batch_rooflines(models, batch, reps, secs):
+    """VALU and HBM fractions of a batch run over `secs` seconds: lane_ops of
+    every model from its own structure and per-simulation counters, plus its
+    inputs, samples and per-simulation records, summed over the batch."""
+    work, hbm = 0.0, 0.0
+    for i, m in enumerate(models):
+        info = m.info()
+        sd = m.structure()
+        SV = int(len(np.unique(sd["rav_flat"][:, 1]))) if info.nV else 0
+        ss = batch.sim_stats(i)
+        st = {k: int(ss[k].sum()) for k in ("leaps", "switches", "poisson", "gaps", "rejections", "ssa_steps")}
+        work += lane_ops(st, info.S, info.R, info.nL, info.nV, info.nF, SV)
+        hbm += int(reps[i]) * (4 * info.S + 4 * info.SS * info.K + 64)
+    return {"valu": {"achieved": work / secs / 1e12, "peak": VALU_PEAK / 1e12, "unit": "Tlane-op/s",
+                     "frac": work / secs / VALU_PEAK},
+            "hbm": {"achieved_GBps": hbm / secs / 1e9, "peak_GBps": HBM_PEAK / 1e9, "frac": hbm / secs / HBM_PEAK}}
+
+
 def model_rooflines(model, stats, kernel_ms, nsims):
     """VALU and HBM fractions of a single-model run: algorithmic lane-ops
     (lane_ops) and bytes (inputs, samples and per-simulation record) over the
@@ -438,8 +456,10 @@ def bench_c2(args):
             fir += st["firings"]
             print(f"# step {k}: firings={st['firings']} kernel_ms={st['kernel_ms']:.1f} launches={st['launches']}",
                   file=sys.stderr, flush=True)
+        # the last step's per-model counters over its wall time (the classes run side by side)
+        roof = batch_rooflines(models, b, reps, (ms / max(args.steps, 1)) / 1e3)
     sims = sum(reps)
-    extra = {}
+    extra = {"roofline": roof}
     if not args.no_cpu:
         extra["cpu_baseline"] = c2_cpu_baseline(nets, ts)
     _line("reaction firings/sec on 1 GPU (batched uneven models)", fir / (ms / 1e3), "firings/s", args.steps,
@@ -460,12 +480,10 @@ def bench_c2(args):
         m.close()
 
 
-def c2_cpu_baseline(nets, ts, every=512, per_model=16):
+def c2_cpu_baseline(nets, ts, every=128, per_model=16):
     """The oracle's REFSEM restatement over a sample of the batch: every
-    512th model, its first min(replicas, 16) simulations, OpenMP over the
-    simulations of a model on the host cores (about 20 s: the reference
-    algorithm recomputes all propensities at every SSA step, which the
-    large models of the batch make slow)."""
+    128th model (64 models), its first min(replicas, 16) simulations,
+    OpenMP over the simulations of a model on the host cores (about 10 s)."""
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
     import oracle

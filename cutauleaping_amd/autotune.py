@@ -321,18 +321,21 @@ def tune(left, right, x0, c, ts, *, feed=None, cs=None, gpu=0, probe_sims=65536,
     return res
 
 
-def tune_batch(nets, replicas, *, gpu=0, probe_fraction=0.1, max_probe_replicas=8192, seed=0x7E58,
-               verbose=False, log=None) -> dict:
+def tune_batch(nets, replicas, *, gpu=0, probe_fraction=None, min_probe_ms=300.0, max_probe_replicas=8192,
+               seed=0x7E58, verbose=False, log=None) -> dict:
     """Lanes per simulation for every model of a batch (for Batch(...,
     lanes=...)).  nets: dicts with left, right, x0, c, ts (and optionally
     feed, cs), replicas: per model.  The models are grouped by the library's
     size rule (the kernel class each would run in); for each class the model
     with the most estimated work (replicas x reactions) is probed as a
-    one-model batch over the first probe_fraction of its horizon, with at
-    most max_probe_replicas replicas, at the class's G and at G/2 and 2G
-    (where the model fits), and the fastest G is applied to every model of
-    the class that fits it.  Returns {"lanes": int32 array, "classes":
-    {rule G: {"model", "scores", "chosen"}}}."""
+    one-model batch with at most max_probe_replicas replicas, at the class's
+    G and at G/2 and 2G (where the model fits), and the fastest G is applied
+    to every model of the class that fits it.  The probe runs the first
+    probe_fraction of the model's horizon; by default (None) the fraction
+    starts at 1/16 and grows (x4 or more) until the probe at the class's G
+    takes min_probe_ms or covers the whole horizon, so that short runs are
+    probed whole.  Returns {"lanes": int32 array, "classes": {rule G:
+    {"model", "scores", "chosen", "fraction"}}}."""
     say = log or ((lambda m: print(m, flush=True)) if verbose else (lambda m: None))
     reps = np.asarray(replicas, np.int64)
 
@@ -356,26 +359,38 @@ def tune_batch(nets, replicas, *, gpu=0, probe_fraction=0.1, max_probe_replicas=
         n = nets[rep]
         S, R = np.asarray(n["left"]).shape[1], np.asarray(n["left"]).shape[0]
         ts = np.asarray(n["ts"], np.float64)
-        t_probe = ts[ts <= ts[-1] * probe_fraction + 1e-12]
-        if len(t_probe) == 0:
-            t_probe = ts[:1]
         nrep = int(min(reps[rep], max_probe_replicas))
         cand = [g for g in (G // 2, G, 2 * G) if 1 <= g <= 64]
         scores = {}
-        m = model_of(n, ts=t_probe)
-        try:
-            for g in cand:
-                try:
-                    with Batch([m], [nrep], gpu=gpu, seed=seed, lanes=[g]) as b:
-                        b.run()
-                        st = b.stats()
-                except CtlError as e:             # a G the model's LDS does not allow
-                    say(f"# tune_batch class {G}: G={g} not possible ({e})")
-                    continue
-                scores[g] = st["firings"] / max(st["kernel_ms"], 1e-3) * 1e3
-                say(f"# tune_batch class {G} (model {rep}, N={S}, M={R}): G={g} {scores[g]:.3e} firings/s")
-        finally:
-            m.close()
+
+        def probe(g, frac):
+            t_probe = ts[ts <= ts[-1] * frac * (1 + 1e-9)]
+            m = model_of(n, ts=t_probe if len(t_probe) else ts[:1])
+            try:
+                with Batch([m], [nrep], gpu=gpu, seed=seed, lanes=[g]) as b:
+                    b.run()
+                    st = b.stats()
+            finally:
+                m.close()
+            return st["firings"] / max(st["kernel_ms"], 1e-3) * 1e3, st["kernel_ms"]
+
+        frac = probe_fraction if probe_fraction is not None else 1.0 / 16
+        rate, ms = probe(G, frac)
+        while probe_fraction is None and ms < min_probe_ms and frac < 1.0:
+            frac = min(1.0, frac * max(4.0, 1.2 * min_probe_ms / max(ms, 1e-3)))
+            rate, ms = probe(G, frac)
+        scores[G] = rate
+        say(f"# tune_batch class {G} (model {rep}, N={S}, M={R}, {frac:g} of the horizon): G={G} {rate:.3e} firings/s")
+        for g in cand:
+            if g == G:
+                continue
+            try:
+                scores[g], _ = probe(g, frac)
+            except CtlError as e:             # a G the model's LDS does not allow
+                say(f"# tune_batch class {G}: G={g} not possible ({e})")
+                continue
+            say(f"# tune_batch class {G} (model {rep}, N={S}, M={R}, {frac:g} of the horizon): G={g} "
+                f"{scores[g]:.3e} firings/s")
         chosen = max(scores, key=scores.get) if scores else G
         if chosen != G:
             # a group's LDS grows with species and reactions, so the chosen G
@@ -384,5 +399,5 @@ def tune_batch(nets, replicas, *, gpu=0, probe_fraction=0.1, max_probe_replicas=
                 Ri, Si = np.asarray(nets[i]["left"]).shape
                 if chosen == 1 or (Si <= S and Ri <= R):
                     lanes[i] = chosen
-        out[G] = {"model": rep, "scores": scores, "chosen": int(chosen)}
+        out[G] = {"model": rep, "scores": scores, "chosen": int(chosen), "fraction": frac}
     return {"lanes": lanes, "classes": out}

@@ -237,9 +237,10 @@ def test_autotune_picks_a_measured_candidate(gpu, tmp_path):
 
 def test_autotune_batch_classes(gpu, oracle_lib):
     """tune_batch probes each kernel class's largest model at its G and the
-    neighbouring G, and returns lanes per model that the batch accepts; the
-    batch then runs each model with those lanes, bit-exact against the
-    oracle's algorithm for that G."""
+    neighbouring G (over a fixed fraction of the horizon, or one grown until
+    the probe is long enough), and returns lanes per model that the batch
+    accepts; the batch then runs each model with those lanes, bit-exact
+    against the oracle's algorithm for that G."""
     from cutauleaping_amd import autotune, networks
     print("case: autotune batch classes", flush=True)
     ts = networks.sample_times(20, 0.05)
@@ -255,6 +256,10 @@ def test_autotune_batch_classes(gpu, oracle_lib):
     assert lanes.shape == (len(nets),) and set(int(g) for g in lanes) <= {1, 2, 4, 8, 16, 32, 64}
     for G, cl in res["classes"].items():
         assert cl["chosen"] in cl["scores"] and cl["scores"][cl["chosen"]] == max(cl["scores"].values())
+        assert cl["fraction"] == 0.5
+    auto = autotune.tune_batch(nets, reps, min_probe_ms=5.0, verbose=True)    # horizon grown from 1/16
+    for G, cl in auto["classes"].items():
+        assert 1.0 / 16 <= cl["fraction"] <= 1.0 and cl["chosen"] in cl["scores"]
     models = [gpu.Model.from_arrays(n["left"], n["right"], n["x0"], n["c"], ts, cs=n["cs"]) for n in nets]
     with gpu.Batch(models, reps, seed=0xB00, lanes=lanes) as b:
         b.run()
