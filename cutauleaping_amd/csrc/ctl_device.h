@@ -88,7 +88,8 @@ __device__ __forceinline__ uint32_t rng_u32(Rng& g) {
 }
 
 // U(0,1] and U[0,1) on the 2^-24 grid.
-__device__ __forceinline__ float rng_oc(Rng& g) { return (float)((rng_u32(g) >> 8) + 1u) * 0x1p-24f; }
+__device__ __forceinline__ float u32_oc(uint32_t r) { return (float)((r >> 8) + 1u) * 0x1p-24f; }
+__device__ __forceinline__ float rng_oc(Rng& g) { return u32_oc(rng_u32(g)); }
 __device__ __forceinline__ float rng_co(Rng& g) { return (float)(rng_u32(g) >> 8) * 0x1p-24f; }
 
 // ---------------------------------------------------------------------------
@@ -138,8 +139,7 @@ __device__ __forceinline__ float dexpf(float x) {
 // The two cases share one series and one division without a branch (the
 // lanes of a wavefront draw different V): f and the exponent are selected,
 // and each case ends with exactly the additions of its own formula.
-__device__ __forceinline__ float exp_rv(Rng& g) {
-    const uint32_t r = rng_u32(g);
+__device__ __forceinline__ float u32_exp(uint32_t r) {
     const float v = r < 0x01000000u ? (float)r * 0x1p-32f : (float)(r >> 8) * 0x1p-24f;
     const bool small = v < 0x1.2bec34p-2f;
     // dlogf's reduction of x = 1 - v (exact on the large branch)
@@ -161,6 +161,7 @@ __device__ __forceinline__ float exp_rv(Rng& g) {
     const float llarge = ef * 0x1.62e4p-1f + (f - (hfsq - (t + ef * 0x1.7f7d1cp-20f)));
     return -(small ? lsmall : llarge);
 }
+__device__ __forceinline__ float exp_rv(Rng& g) { return u32_exp(rng_u32(g)); }
 
 // ---------------------------------------------------------------------------
 // Poisson variates (the reference calls curand_poisson, T.cu:2018, 2040).

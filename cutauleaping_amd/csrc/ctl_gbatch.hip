@@ -28,7 +28,6 @@ __global__ __launch_bounds__(64) void gbatch_kernel(GBatchParams bp) {
     L.ssa_left = 0;
     L.ssa_mode = 0;
     L.slot = slot;
-    L.gg.tag = 0u;
     L.lg.tag = 1u + (uint32_t)l;
     int m = __builtin_amdgcn_readfirstlane(bp.slot_model[slot]);
     L.sim = __builtin_amdgcn_readfirstlane(bp.slot_sim[slot]);
@@ -54,7 +53,8 @@ __global__ __launch_bounds__(64) void gbatch_kernel(GBatchParams bp) {
         L.gg.k0 = L.lg.k0 = p.key0;
         L.gg.k1 = L.lg.k1 = p.key1;
         L.gg.sim = L.lg.sim = p.sim_base + (uint32_t)L.sim;
-        L.gg.pos = L.lg.pos = 4;
+        L.gg.cb = GRNG_NONE;
+        L.lg.pos = 4;
         for (int s = l; s < p.S; s += G) V.Xb[s] = bp.X[(size_t)slot * bp.maxS + s];
         L.crate = grates(p, L.sim);
         L.fvrow = p.nfs > 0 ? p.fv + (size_t)(L.sim / p.fv_div) * p.nfs : nullptr;
@@ -88,7 +88,8 @@ __global__ __launch_bounds__(64) void gbatch_kernel(GBatchParams bp) {
             L.gg.k0 = L.lg.k0 = p.key0;
             L.gg.k1 = L.lg.k1 = p.key1;
             L.gg.sim = L.lg.sim = p.sim_base + r;
-            L.gg.pos = L.lg.pos = 4;
+            L.gg.cb = GRNG_NONE;
+            L.lg.pos = 4;
             for (int s = l; s < p.S; s += G)
                 V.Xb[s] = (int32_t)(p.x0_bcast ? p.x0_bcast[s] : p.x0_sims[(size_t)(r / (uint32_t)p.x0_div) * p.S + s]);
             L.crate = grates(p, (int)r);
@@ -150,7 +151,6 @@ void gclass_kernel(GClassParams bp) {
     L.ssa_left = 0;
     L.ssa_mode = 0;
     L.slot = wave * (64 / G) + gib;
-    L.gg.tag = 0u;
     L.lg.tag = 1u + (uint32_t)l;
     int m = __builtin_amdgcn_readfirstlane(bp.wave_model[wave]);
     L.sim = m >= 0 ? bp.slot_sim[L.slot] : -1;
@@ -175,7 +175,8 @@ void gclass_kernel(GClassParams bp) {
             L.poisson = bp.poisson[(size_t)s0 * G + l];
             L.gaps = bp.gaps[(size_t)s0 * G + l];
             L.gg.sim = L.lg.sim = p.sim_base + (uint32_t)L.sim;
-            L.gg.pos = L.lg.pos = 4;
+            L.gg.cb = GRNG_NONE;
+            L.lg.pos = 4;
             for (int s = l; s < p.S; s += G) V.Xb[s] = bp.X[(size_t)s0 * bp.maxS + s];
             L.crate = grates(p, L.sim);
             L.fvrow = p.nfs > 0 ? p.fv + (size_t)(L.sim / p.fv_div) * p.nfs : nullptr;

@@ -292,6 +292,50 @@ __device__ __forceinline__ uint64_t gsum64(uint64_t v) {
 
 // Group state: the scalars are identical in every lane of the group; the
 // Poisson / gap counters are per lane and summed when the simulation ends.
+// The group's stream (counter word 3 = 0) with its Philox blocks spread over
+// the group: lane i holds block cb + i of the current event, so 4G words (2G
+// SSA steps) cost each lane one Philox evaluation where a per-lane buffer
+// costs one per 4 words, and the groups of a wavefront, at different points
+// of their streams, seldom make the wavefront evaluate Philox for one of them
+// alone.  A word is read from the lane that holds its block; the words and
+// their order are those of a per-lane stream (rng_u32).
+struct GRng {
+    uint32_t k0, k1;             // key
+    uint32_t sim;                // global simulation index
+    uint32_t ev;                 // current event
+    uint32_t wc;                 // words of the event consumed
+    uint32_t cb;                 // first block held by the group (GRNG_NONE: none)
+    uint32_t c0, c1, c2, c3;     // this lane's block, cb + lane
+};
+constexpr uint32_t GRNG_NONE = 0x80000000u;   // b - cb >= 2^31 for every block b of an event
+
+__device__ __forceinline__ void grng_event(GRng& g, uint32_t ev) {
+    g.ev = ev;
+    g.wc = 0u;
+    g.cb = GRNG_NONE;
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t grng_u32(GRng& g, int l) {
+    const uint32_t b = g.wc >> 2;
+    if (b - g.cb >= (uint32_t)G) {
+        g.cb = b;
+        philox_block(b + (uint32_t)l, g.ev, g.sim, 0u, g.k0, g.k1, g.c0, g.c1, g.c2, g.c3);
+    }
+    // the four words as values: a select between loads of the struct's fields
+    // would become a load through a selected address, which keeps the whole
+    // lane state in scratch memory
+    uint32_t c0 = g.c0, c1 = g.c1, c2 = g.c2, c3 = g.c3;
+    asm volatile("" : "+v"(c0), "+v"(c1), "+v"(c2), "+v"(c3));
+    const uint32_t k = g.wc & 3u;
+    const uint32_t lo = (k & 1u) ? c1 : c0;
+    const uint32_t hi = (k & 1u) ? c3 : c2;
+    const uint32_t mine = (k & 2u) ? hi : lo;
+    const int src = (int)(b - g.cb);
+    g.wc += 1u;
+    return __shfl(mine, src, G);
+}
+
 struct GLane {
     int slot;                // group slot
     int sim;                 // local index, -1 none, -2 queue exhausted
@@ -302,7 +346,7 @@ struct GLane {
     uint64_t poisson, gaps;
     const float* crate;      // the simulation's rate constants (HBM, cached)
     const uint32_t* fvrow;   // the simulation's feed values (when the model has feeds)
-    Rng gg;                  // group stream (word 3 = 0)
+    GRng gg;                 // group stream (word 3 = 0)
     Rng lg;                  // this lane's stream (word 3 = 1 + lane)
     double tacc;             // clock advance of the SSA burst in progress
     int ssa_left;            // steps left in the SSA burst in progress (0: none)
@@ -695,13 +739,13 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
     CTL_PHASE(PH_TAU1);
     if (tau1 < p.ssa_factor / a0) { L.switches++; return LEAP_SWITCH; }      // T.cu:1935-1941
     uint32_t e = L.evc++;
-    rng_event(L.gg, e);
+    grng_event(L.gg, e);
     float tau2 = 3.40282347e+38f;
-    if (a0c > 0.0f) tau2 = exp_rv(L.gg) / a0c;                                // T.cu:1962-1965
+    if (a0c > 0.0f) tau2 = u32_exp(grng_u32<G>(L.gg, l)) / a0c;                                // T.cu:1962-1965
     float tau = tau1 < tau2 ? tau1 : tau2;
     int pick = -1;
     if (tau1 > tau2) {                                                         // T.cu:1993-2006
-        const float w = rng_oc(L.gg) * a0c;
+        const float w = u32_oc(grng_u32<G>(L.gg, l)) * a0c;
         float Q = 0.0f;
         for (int j = l; j < p.R; j += G) {
             const uint32_t u = V.A[j];
@@ -770,7 +814,10 @@ __device__ __forceinline__ int gleap(const KParams& p, GLane& L, const GView& V,
         for (uint32_t qi = (uint32_t)l; qi < nq; qi += G) {
             const int j = Q[qi];
             const float lam = tau * __uint_as_float(V.A[j]);
-            Rng gr = L.gg;
+            Rng gr{};
+            gr.k0 = L.gg.k0;
+            gr.k1 = L.gg.k1;
+            gr.sim = L.gg.sim;
             gr.tag = 0x80000000u | (uint32_t)j;
             rng_event(gr, e);
             const int32_t k = poisson(gr, lam, recip);
@@ -860,7 +907,7 @@ template <int G, int PFM = 3>
 __device__ __forceinline__ bool gssa_run(const KParams& p, GLane& L, const GView& V, int l, int nmax) {
     int32_t* X = V.Xb + (L.xb ? p.S : 0);
     if (L.ssa_left == 0) {
-        rng_event(L.gg, L.evc++);
+        grng_event(L.gg, L.evc++);
         L.tacc = 0.0;
         L.ssa_left = p.ssa_steps;
         if (pmode<PFM>(kuse(p.pfast)) == 1) {
@@ -888,10 +935,10 @@ __device__ __forceinline__ bool gssa_run(const KParams& p, GLane& L, const GView
         P = gscan<G>(P, l);
         const float a0 = gbcast_last<G>(P, l);
         if (a0 == 0.0f) { gabsorb<G>(p, L, X, l); L.ssa_left = 0; return true; }
-        const float r1 = rng_oc(L.gg) * a0;
+        const float r1 = u32_oc(grng_u32<G>(L.gg, l)) * a0;
         const int j = gselect<G>(p, V.A, Pown, P, r1, l, 0);
         if (j < 0) { gabsorb<G>(p, L, X, l); L.ssa_left = 0; return true; }    // unreachable for finite a0
-        const float dt = exp_rv(L.gg) / a0;
+        const float dt = u32_exp(grng_u32<G>(L.gg, l)) / a0;
         CTL_PHASE(PH_SSA_SELECT);
         tacc = tacc + (double)dt;
         const double tn = L.t + tacc;
@@ -995,7 +1042,8 @@ __device__ __forceinline__ bool gfetch(const KParams& p, GLane& L, const GView& 
     L.leaps = L.rejections = L.switches = L.absorbed = 0;
     L.poisson = L.gaps = 0;
     L.gg.sim = L.lg.sim = p.sim_base + q;
-    L.gg.pos = L.lg.pos = 4;
+    L.gg.cb = GRNG_NONE;
+    L.lg.pos = 4;
     for (int s = l; s < p.S; s += G)
         V.Xb[s] = (int32_t)(p.x0_bcast ? p.x0_bcast[s] : p.x0_sims[(size_t)(q / (uint32_t)p.x0_div) * p.S + s]);
     L.crate = grates(p, (int)q);
@@ -1067,13 +1115,12 @@ __device__ __forceinline__ void gstage_put(const KParams& p, const GLane& L, int
         p.st->rejections[s0] = L.rejections;
         p.st->switches[s0] = L.switches;
         p.st->ev[s0] = L.gg.ev;
-        p.st->blk[s0] = L.gg.blk;
-        p.st->pos[s0] = L.gg.pos;
+        p.st->blk[s0] = L.gg.wc;      // words of the event consumed
     }
 }
 
 // Takes over the simulation of slot s0: its registers, rate and feed rows and
-// group stream, whose partly consumed block is regenerated.
+// the position in its group stream.
 template <int G>
 __device__ __forceinline__ void gstage_get(const KParams& p, GLane& L, int s0, int l) {
     L.sim = p.st->sim[s0];
@@ -1094,20 +1141,11 @@ __device__ __forceinline__ void gstage_get(const KParams& p, GLane& L, int s0, i
     L.rejections = p.st->rejections[s0];
     L.switches = p.st->switches[s0];
     L.gg.ev = p.st->ev[s0];
-    L.gg.blk = p.st->blk[s0];
-    L.gg.pos = p.st->pos[s0];
+    L.gg.wc = p.st->blk[s0];
+    L.gg.cb = GRNG_NONE;                 // the blocks are regenerated at the next word
     L.gg.sim = L.lg.sim = p.sim_base + (uint32_t)L.sim;
     L.crate = grates(p, L.sim);
     L.fvrow = p.nfs > 0 ? p.fv + (size_t)(L.sim / p.fv_div) * p.nfs : nullptr;
-    if (L.gg.pos < 4u) {                 // the block in use: regenerate it, drop the words consumed
-        philox_block(L.gg.blk - 1u, L.gg.ev, L.gg.sim, L.gg.tag, L.gg.k0, L.gg.k1, L.gg.b0, L.gg.b1, L.gg.b2,
-                     L.gg.b3);
-        for (uint32_t i = 0; i < L.gg.pos; ++i) {
-            L.gg.b0 = L.gg.b1;
-            L.gg.b1 = L.gg.b2;
-            L.gg.b2 = L.gg.b3;
-        }
-    }
 }
 
 // Stable partition of the workgroup's regions by key (0 leap, 1 SSA, 2
@@ -1179,7 +1217,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G
     gzero_pad<G>(V, p.R, l);
     L.gg.k0 = L.lg.k0 = p.key0;
     L.gg.k1 = L.lg.k1 = p.key1;
-    L.gg.tag = 0u;
     L.lg.tag = 1u + (uint32_t)l;
     L.tacc = 0.0;
     L.ssa_left = 0;
@@ -1206,7 +1243,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 8 || G
         L.poisson = p.st->poisson[(size_t)s0 * G + l];
         L.gaps = p.st->gaps[(size_t)s0 * G + l];
         L.gg.sim = L.lg.sim = p.sim_base + (uint32_t)L.sim;
-        L.gg.pos = L.lg.pos = 4;
+        L.gg.cb = GRNG_NONE;
+        L.lg.pos = 4;
         for (int s = l; s < p.S; s += G) V.Xb[s] = p.X[(size_t)s0 * p.S + s];
         L.crate = grates(p, L.sim);
         L.fvrow = p.nfs > 0 ? p.fv + (size_t)(L.sim / p.fv_div) * p.nfs : nullptr;
