@@ -579,62 +579,39 @@ __device__ __forceinline__ int gselect(const KParams& p, const uint32_t* A, floa
     const uint64_t hit = gballot<G>(Pown > 0.0f && P >= w);
     const int sel = hit ? __builtin_ctzll(hit) : 63 - __builtin_clzll(cand);
     const float excl = G <= 16 ? dppf<DPP_ROW_SHR1>(P) : dppf<DPP_WAVE_SHR1>(P);   // lane 0's value is unused
-    if constexpr (G <= 16) {
-        // The walk through lane sel's reactions sel + i*G, spread over the
-        // group: lane i holds the i-th propensity and the sequential sums
-        // excl + a_0 + ... + a_i are built by G-1 row shifts (after k shifts
-        // lanes 0..k hold their final sum; a shifted-in 0 or a skipped entry
-        // adds an exact 0), so every sum is rounded as in the one-lane walk.
-        const int nj = (p.R - sel + G - 1) / G;
-        float carry = __shfl(excl, sel, G);
-        if (sel == 0) carry = 0.0f;
-        int last = -1;
-        for (int c = 0; c < nj; c += G) {
-            const int i = c + l;
-            const int j = sel + i * G;
-            const uint32_t u = i < nj ? A[j] : 0u;
-            const bool use = which != 2 || (u & 0x80000000u) || j == 0;
-            const float a = use ? aval(u) : 0.0f;
-            const float a1 = l == 0 ? carry + a : a;
-            float x = a1;
+    // The walk through lane sel's reactions sel + i*G, spread over the first
+    // W lanes of the group (a row of 16 at most): lane i holds the i-th
+    // propensity and the sequential sums excl + a_0 + ... + a_i are built by
+    // W-1 row shifts (after k shifts lanes 0..k hold their final sum; a
+    // shifted-in 0 or a skipped entry adds an exact 0), so every sum is
+    // rounded as in the one-lane walk.  Lane sel's reactions number at most
+    // 16 per round; groups wider than 16 leave their other rows idle.
+    constexpr int W = G <= 16 ? G : 16;
+    const int nj = (p.R - sel + G - 1) / G;
+    float carry = __shfl(excl, sel, G);
+    if (sel == 0) carry = 0.0f;
+    int last = -1;
+    for (int c = 0; c < nj; c += W) {
+        const int i = c + l;
+        const bool in = l < W && i < nj;
+        const int j = sel + i * G;
+        const uint32_t u = in ? A[j] : 0u;
+        const bool use = which != 2 || (u & 0x80000000u) || j == 0;
+        const float a = use ? aval(u) : 0.0f;
+        const float a1 = l == 0 ? carry + a : a;
+        float x = a1;
 #pragma unroll
-            for (int k = 1; k < G; ++k) {
-                const float t = dppf<DPP_ROW_SHR1>(x) + a1;
-                x = (G == 16 || l > 0) ? t : a1;
-            }
-            const uint64_t pos = gballot<G>(i < nj && a > 0.0f);
-            const uint64_t hm = gballot<G>(i < nj && a > 0.0f && w <= x);
-            if (hm) return sel + (c + __builtin_ctzll(hm)) * G;
-            if (pos) last = sel + (c + 63 - __builtin_clzll(pos)) * G;
-            carry = gbcast_last<G>(x, l);
+        for (int k = 1; k < W; ++k) {
+            const float t = dppf<DPP_ROW_SHR1>(x) + a1;
+            x = (W == 16 || l > 0) ? t : a1;
         }
-        return last;
+        const uint64_t pos = gballot<G>(in && a > 0.0f);
+        const uint64_t hm = gballot<G>(in && a > 0.0f && w <= x);
+        if (hm) return sel + (c + __builtin_ctzll(hm)) * G;
+        if (pos) last = sel + (c + 63 - __builtin_clzll(pos)) * G;
+        carry = G <= 16 ? gbcast_last<G>(x, l) : __shfl(x, W - 1, G);
     }
-    int pick = -1;
-    if (l == sel) {
-        float acc = l > 0 ? excl : 0.0f;
-        int last = -1;
-        // four propensities per batch of loads; past the end reads as 0,
-        // which neither moves acc nor can be chosen
-        for (int j0 = l; j0 < p.R && pick < 0; j0 += 4 * G) {
-            uint32_t u[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) u[i] = j0 + i * G < p.R ? A[j0 + i * G] : 0u;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int j = j0 + i * G;
-                const bool use = which != 2 || (u[i] & 0x80000000u) || j == 0;
-                const float a = use ? aval(u[i]) : 0.0f;
-                acc = acc + a;
-                if (pick < 0 && a > 0.0f) {
-                    last = j;
-                    if (w <= acc) pick = j;
-                }
-            }
-        }
-        if (pick < 0) pick = last;
-    }
-    return __shfl(pick, sel, G);
+    return last;
 }
 
 template <int G>
