@@ -63,7 +63,7 @@ import time
 
 import numpy as np
 
-from . import Batch, CtlError, Model, Sim
+from . import CTL_E_LIMIT, Batch, CtlError, Model, Sim
 
 CANDIDATES = (1, 2, 4, 8, 16, 32, 64)
 _memo: dict = {}
@@ -305,6 +305,8 @@ def tune(left, right, x0, c, ts, *, feed=None, cs=None, gpu=0, probe_sims=65536,
     finally:
         for m in models:
             m.close()
+    if best is None:
+        raise CtlError(CTL_E_LIMIT, f"autotune: none of the candidates {tuple(candidates)} runs this model")
     res = {"lanes_per_sim": int(best["lanes_per_sim"]), "options": best, "scores": scores, "confirm": confirmed,
            "probe": {"sims": probe_sims, "windows": W, "t_end": T / W, "window_starts": [T * w / W for w in range(W)],
                      "calibration": calib, "confirm_windows": W_conf if confirmed else None,
