@@ -35,10 +35,13 @@ actual device and keeps the fastest configuration:
 4. Stage 4 times the rivals on fewer, longer windows -- the whole horizon
    when the estimate fits `confirm_ms` -- and keeps the fastest: the best
    configuration, the library's defaults at its G, the best one of its G on
-   the other side of regrouping, and the runner-up G of stage 1 when it came
-   within 10 %.  Short windows cut
-   simulations off their neighbours' regimes, and the decision that depends
-   most on that mixing (regrouping) is the one confirmed.
+   the other side of regrouping, and the defaults at every other G that came
+   within 25 % in stage 1.  Short windows cut simulations off their
+   neighbours' regimes and time many short simulations instead of a few long
+   ones, which narrows the gaps between G (on the 100x200 network the probe
+   put G = 16 and 32 within 1 % and G = 64 within 6 %; whole runs give 7.4
+   and 5.5e9 firings/s at G = 16 and 32), and misjudges regrouping, which
+   depends on that mixing.
 
 A configuration's score is its firings over its kernel time (firings per
 second of the probe launch).  Results are
@@ -269,7 +272,8 @@ def tune(left, right, x0, c, ts, *, feed=None, cs=None, gpu=0, probe_sims=65536,
             # Stage 4: the rivals run longer windows (the whole horizon when
             # that fits confirm_ms): the best configuration, the library's
             # defaults at its G, the best one of its G on the other side of
-            # regrouping, and the runner-up G of stage 1 when it came within 10 %.
+            # regrouping, and the defaults at every other G that came within
+            # 25 % in stage 1.
             def regroups(o):
                 ci = o.get("compact_interval", 0)
                 return ci > 0 or (ci == 0 and 16 <= o["lanes_per_sim"] < 64 and not force_ssa)
@@ -284,9 +288,9 @@ def tune(left, right, x0, c, ts, *, feed=None, cs=None, gpu=0, probe_sims=65536,
                             other = o
                 if other is not None:
                     rivals.append(other)
-            runner = sorted((r, g) for g, r in stage1.items() if g != G and r > 0)
-            if runner and runner[-1][0] >= 0.9 * stage1[G]:
-                rivals.append({"lanes_per_sim": runner[-1][1]})
+            for g, r in sorted(stage1.items()):
+                if g != G and r >= 0.75 * stage1[G]:
+                    rivals.append({"lanes_per_sim": g})
             probe_ms = fir / max(best_rate, 1e-9) * 1e3      # the default's firings at the best rate
             W_conf = next((d for d in range(1, W) if W % d == 0 and probe_ms * W / d <= confirm_ms), None)
             if confirm_ms > 0 and len(rivals) > 1 and W_conf is not None:

@@ -204,7 +204,7 @@ def test_autotune_picks_a_measured_candidate(gpu, tmp_path):
     longer windows until the launch is long enough), refines the winner's
     regrouping interval, SSA chunk and iteration budget, confirms the
     best configuration against its rivals (the library's defaults, the other
-    side of regrouping, a close runner-up G) on longer windows, and returns the fastest
+    side of regrouping, other G within 25 %) on longer windows, and returns the fastest
     configuration measured; the result is memoised on disk."""
     from cutauleaping_amd import autotune, networks
     print("case: autotune", flush=True)
@@ -219,7 +219,7 @@ def test_autotune_picks_a_measured_candidate(gpu, tmp_path):
     assert min(stage1) > 0
     assert len(res["scores"]) > 4                          # stage 2 ran
     if res["confirm"]:                                     # stage 4: the longer windows decide
-        assert 2 <= len(res["confirm"]) <= 4
+        assert len(res["confirm"]) >= 2
         assert res["confirm"][autotune._key(res["options"])] == max(res["confirm"].values())
         assert res["probe"]["windows"] % res["probe"]["confirm_windows"] == 0
     else:

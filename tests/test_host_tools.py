@@ -132,7 +132,7 @@ def _fake_sim(rate_of, ms_per_unit):
 def test_tune_control_flow(monkeypatch, tmp_path):
     """Calibration cuts the windows until the probe is long enough; stage 4
     re-times the best configuration, the library's defaults at its G, the
-    other side of regrouping and a close runner-up G over the whole horizon,
+    other side of regrouping and every G within 25 % over the whole horizon,
     and its fastest wins even against the probe's ranking; the result is
     memoised on disk."""
     G_RATE = {1: 1e8, 2: 2e8, 4: 4e8, 8: 8e8, 16: 1.2e9, 32: 1.15e9, 64: 5e8}
@@ -174,7 +174,7 @@ def test_tune_control_flow(monkeypatch, tmp_path):
     conf = res["confirm"]
     assert "lanes_per_sim=16" in conf                           # the defaults at the best G
     assert "compact_interval=-1,lanes_per_sim=16" in conf       # the probe's favourite
-    assert "lanes_per_sim=32" in conf                           # runner-up within 10 %
+    assert "lanes_per_sim=32" in conf                           # another G within 25 %
     assert res["options"] == {"lanes_per_sim": 16}              # the whole horizon decides
     assert all(conf[k] == pytest.approx(whole[k], rel=1e-9) for k in conf) and len(conf) == 4
     autotune._memo.clear()
