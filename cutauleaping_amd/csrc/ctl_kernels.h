@@ -35,8 +35,8 @@ struct SlotState {          // SoA, one entry per slot
     int32_t* mode;
     uint32_t* evc;          // next event number
     uint32_t* ev;           // event of the SSA burst in progress
-    uint32_t* blk;
-    uint32_t* pos;
+    uint32_t* blk;          // one lane: next Philox block; group kernels: words of the event consumed
+    uint32_t* pos;          // one lane: words of the buffered block used (group kernels: unused)
     int32_t* ssa_left;      // steps left in the current SSA burst (0 = none)
     uint32_t* xb;           // parity of the current state buffer (global storage)
     uint64_t* firings;
