@@ -480,10 +480,11 @@ def bench_c2(args):
         m.close()
 
 
-def c2_cpu_baseline(nets, ts, every=128, per_model=16):
+def c2_cpu_baseline(nets, ts, every=64, per_model=16):
     """The oracle's REFSEM restatement over a sample of the batch: every
-    128th model (64 models), its first min(replicas, 16) simulations,
-    OpenMP over the simulations of a model on the host cores (about 10 s)."""
+    64th model (128 models), its first min(replicas, 16) simulations,
+    OpenMP over the simulations of a model on the host cores (about 7 s;
+    every 128th model took 3.4 s)."""
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
     import oracle
